@@ -1,0 +1,172 @@
+"""Benchmark: MC photons propagated per second on one MI355X.
+
+Workload (one step = one pp_propagate of the whole event): cascade_16_strings,
+a 10 TeV electron cascade of 1e8 photons inside a 16-string, 320-module array
+(4 x 4 strings at 50 m, 20 spheres of r = 0.3 m at 10 m), Cascadia water,
+PMT-acceptance-weighted Cherenkov spectrum 300-700 nm, 15 steps. Inputs
+(targets, spectrum, medium) are resident on the device before timing; hits
+stay on the device (the host copy is not part of a step).
+
+Prints one JSON line. `roofline` is the FP32 VALU roofline of the dominant
+kernel (algorithmic FP32 operations per photon step from the reference step,
+counted in DESIGN.md, over the kernel's measured HIP-event time);
+`cpu_baseline` is the oracle (C restatement of the reference kernel, OpenMP)
+on a bounded sample of the same workload on the host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "photonpropagation.jl_amd"))
+
+# Algorithmic work per photon step of the reference kernel
+# (src/photon_prop_cuda.jl:486-514), FMA = 2 ops:
+#   step length, move, time, phase function mixture, scattering rotation
+W_PHYS_FLOP = 47.0
+W_PHYS_TRANSC = 7.4
+#   one _intersect_nsphere test (src/photon_prop_cuda.jl:309-343)
+W_ISEC_FLOP = 20.0
+W_ISEC_TRANSC = 1.0
+PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # without packed-FP32 instructions
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--photons", type=float, default=1e8)
+    ap.add_argument("--search", choices=["auto", "brute", "grid"], default="auto")
+    ap.add_argument("--cpu-sample", type=float, default=0.0,
+                    help="photons for the CPU baseline (0 = calibrate to ~10 s)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def total_steps(hits, n_ph_sim, n_steps):
+    """Photon steps executed: every photon runs n_steps unless it hits earlier."""
+    import numpy as np
+    return int(n_ph_sim) * n_steps - int(np.sum(n_steps - hits["n_steps"].astype(np.int64)))
+
+
+def cpu_baseline(wl_factory, photons):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import photonprop as P
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+
+    def once(n):
+        wl = wl_factory(n)
+        t = time.perf_counter()
+        O.propagate(P.processing.sources_to_pod(wl.sources), wl.medium.to_pod(),
+                    P.targets_to_pod(wl.targets), wl.seed, n_steps=wl.n_steps,
+                    table=wl.spectrum.table, mapping=wl.spectrum.mapping, n_threads=threads)
+        return time.perf_counter() - t
+
+    if photons <= 0:
+        dt = once(200_000)
+        photons = max(200_000, int(200_000 * 10.0 / max(dt, 1e-3)))
+        photons = min(photons, 20_000_000)
+    once(min(photons, 200_000))  # warm-up
+    runs = sorted(photons / once(photons) for _ in range(3))
+    return dict(value=runs[1], unit="photons/s", cores=threads, kind="port",
+                sample=f"median of 3 oracle runs of {photons:.3g} photons of the same event "
+                       f"(brute-force 320-sphere search, 15 steps, OpenMP {threads} threads)")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design only)"}))
+        sys.exit(2)
+    import ctypes
+
+    import numpy as np
+
+    import photonprop as P
+
+    lib = P._lib.load_library()
+    if lib.pp_device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    nph = int(args.photons)
+    wl = P.workloads.cascade_16_strings(photons=nph)
+    cfg = P._lib.pp_config()
+    cfg.search = {"auto": 0, "brute": 1, "grid": 2}[args.search]
+    cfg.deterministic_order = 0
+    _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
+    P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
+    pods = P.processing.sources_to_pod(wl.sources)
+    n_hits = ctypes.c_int64()
+    n_sim = ctypes.c_int64()
+
+    def step():
+        rc = lib.pp_propagate(dev.handle, pods, 1, ctypes.c_uint64(wl.seed), wl.n_steps, 1.0,
+                              ctypes.byref(n_hits), ctypes.byref(n_sim))
+        P._lib.check(rc, "pp_propagate")
+
+    for _ in range(args.warmup):
+        step()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kernel_ms.append(dev.last_timing()[0])
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    photons_per_s = n_sim.value / (ms_per_step / 1e3)
+
+    hbc = P.HostHitBuffer(np.float32)
+    hits, nsim = P.run_photon_prop_no_local_cache(wl.sources, wl.targets, wl.medium, wl.spectrum,
+                                                  wl.seed, hbc, dev, n_steps=wl.n_steps)
+    steps = total_steps(hits, nsim, wl.n_steps)
+    kms = float(np.mean(kernel_ms))
+    n_t = len(wl.targets)
+    tbar = n_t  # sphere tests per step of the reference's brute-force search
+    flop = steps * (W_PHYS_FLOP + W_ISEC_FLOP * tbar)
+    achieved = flop / (kms * 1e-3) / 1e12
+    transc = steps * (W_PHYS_TRANSC + W_ISEC_TRANSC * tbar) / (kms * 1e-3)
+    out = {
+        "metric": "photons propagated/s on 1 MI355X",
+        "value": photons_per_s,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cascade_16_strings", "photons": nph, "targets": n_t,
+                   "n_steps": wl.n_steps, "search": args.search,
+                   "source": "ExtendedCherenkovEmitter 10 TeV e-, 4x4x20 spheres r=0.3 m"},
+        "photon_steps_per_s": steps / (ms_per_step / 1e3),
+        "kernel_ms": kms,
+        "hits_per_step": int(len(hits["n_steps"])),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None,
+                     "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+                     "transcendental_per_s": transc,
+                     "flop_per_launch": flop,
+                     "basis": "reference-equivalent FP32 ops: (47 + 20*320) per photon step"},
+    }
+    dev.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(
+            lambda n: P.workloads.cascade_16_strings(photons=n), args.cpu_sample)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

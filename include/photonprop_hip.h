@@ -1,0 +1,262 @@
+/*
+ * photonprop_hip.h - C ABI of libphotonprop_hip.so, the MI355X (gfx950) photon
+ * propagation Monte Carlo.
+ *
+ * The library replaces the CUDA.jl propagation path of PhotonPropagation.jl v0.9.3:
+ *   - run_photon_prop_no_local_cache!  (src/photon_prop_cuda.jl:889-959)
+ *   - make_hit_buffers                  (src/photon_prop_cuda.jl:878-886)
+ *   - cuda_propagate_photons!           (src/photon_prop_cuda.jl:429-549)
+ * Everything is plain C: POD structs, raw pointers and sizes. Every function
+ * returns an int status: PP_OK (0), PP_ERR_HIT_OVERFLOW (1, the reference's
+ * PhotonPropOOMException, src/photon_prop_cuda.jl:40, :937-939) or a negative
+ * error code; pp_last_error() gives a message for the calling thread.
+ *
+ * One context per host thread. A context owns its device buffers, its target
+ * and spectrum tables and one HIP stream. Contexts are independent.
+ */
+#ifndef PHOTONPROP_HIP_H
+#define PHOTONPROP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+#define PP_OK 0
+#define PP_ERR_HIT_OVERFLOW 1 /* hit buffer full: PhotonPropOOMException */
+#define PP_ERR_INVALID (-1)   /* bad argument                           */
+#define PP_ERR_HIP (-2)       /* HIP runtime error                      */
+#define PP_ERR_NOMEM (-3)     /* device or host allocation failed       */
+#define PP_ERR_STATE (-4)     /* call sequence error (e.g. no targets)  */
+
+/* ---- emitters: one tagged POD for every PhotonSource subtype ----------
+ * Field layouts of src/lightyield.jl:503-804. Unused fields are ignored. */
+enum pp_source_kind {
+    PP_SRC_POINTLIKE_ISOTROPIC = 0,       /* lightyield.jl:549-553 */
+    PP_SRC_POINTLIKE_TIME_RANGE = 1,      /* lightyield.jl:592-596 */
+    PP_SRC_AXICONE = 2,                   /* lightyield.jl:503-510 */
+    PP_SRC_COLLIMATED_ISOTROPIC = 3,      /* lightyield.jl:570-575 */
+    PP_SRC_PENCIL = 4,                    /* lightyield.jl:527-534 */
+    PP_SRC_EXTENDED_CHERENKOV = 5,        /* lightyield.jl:616-622 */
+    PP_SRC_EXTENDED_CHERENKOV_NO_SMEAR = 6, /* lightyield.jl:670-676 */
+    PP_SRC_EXTENDED_CHERENKOV_CUSTOM_SMEAR = 7, /* lightyield.jl:725-733 */
+    PP_SRC_POINTLIKE_CHERENKOV = 8,       /* lightyield.jl:776-781 */
+    PP_SRC_CHERENKOV_TRACK = 9,           /* lightyield.jl:798-804 */
+    PP_SRC_KIND_COUNT = 10
+};
+
+typedef struct pp_source {
+    int32_t kind;          /* enum pp_source_kind                           */
+    int32_t reserved0;
+    int64_t photons;       /* PhotonSource.photons                          */
+    float position[3];     /* m                                             */
+    float direction[3];    /* unit vector (axis of beam / cascade / track)  */
+    float time;            /* ns; PointlikeTimeRange: time_range[1]         */
+    float time_end;        /* PointlikeTimeRange: time_range[2]             */
+    float angle;           /* Axicone opening angle (rad)                   */
+    float beam_divergence; /* Axicone, Pencil (rad)                         */
+    float cut_angle_cos;   /* CollimatedIsotropic                           */
+    float length;          /* CherenkovTrack length (m)                     */
+    float long_a;          /* LongitudinalParameterisation.a                */
+    float long_b;          /* LongitudinalParameterisation.b                */
+    float long_lrad;       /* LongitudinalParameterisation.lrad (m)         */
+    float beta_a;          /* CustomSmear cherenkov_beta_a                  */
+    float beta_b;          /* CustomSmear cherenkov_beta_b                  */
+    float reserved1[3];
+} pp_source; /* 96 bytes */
+
+/* ---- media -------------------------------------------------------------
+ * CASCADIA: CascadiaMediumProperties (src/water_properties.jl:43-100):
+ *   Kopelevich scattering length, MixedHGES phase function, Quan-Fry
+ *   dispersion. SIMPLE: SimpleMediumProperties (src/simple_medium.jl:3-15),
+ *   constant lengths and an HG phase function. */
+enum pp_medium_kind { PP_MEDIUM_SIMPLE = 0, PP_MEDIUM_CASCADIA = 1 };
+enum pp_phase_kind { PP_PHASE_HG = 0, PP_PHASE_MIXED_HG_ES = 1 };
+
+typedef struct pp_medium {
+    int32_t kind;          /* enum pp_medium_kind                     */
+    int32_t phase_kind;    /* enum pp_phase_kind                      */
+    float hg_g;            /* HG mean cosine                          */
+    float es_b;            /* Einstein-Smoluchowski b                 */
+    float hg_fraction;     /* probability of the HG branch            */
+    float sca_scale;       /* multiplies the scattering length        */
+    float abs_scale;       /* multiplies the absorption length (host) */
+    /* SIMPLE */
+    float sca_len;         /* m                                       */
+    float group_velocity;  /* m/ns                                    */
+    float phase_refractive_index;
+    /* CASCADIA: Kopelevich volume concentrations and Quan-Fry inputs */
+    float vol_conc_small;
+    float vol_conc_large;
+    float salinity;        /* permille                                */
+    float temperature;     /* deg C                                   */
+    float pressure;        /* atm                                     */
+    float reserved[9];
+} pp_medium; /* 96 bytes */
+
+/* ---- targets ----------------------------------------------------------
+ * Index order is kept: it defines the reference's first-hit rule
+ * (src/photon_prop_cuda.jl:497-506). */
+enum pp_shape_kind { PP_SHAPE_SPHERE = 0 };
+
+typedef struct pp_target {
+    float position[3];
+    float radius;
+    int32_t shape_kind;    /* enum pp_shape_kind      */
+    int32_t module_id;     /* carried, not used on device */
+    float reserved[2];
+} pp_target; /* 32 bytes */
+
+/* ---- spectrum -----------------------------------------------------------
+ * Monochromatic (src/spectrum.jl:66-72) or an inverse-CDF table of n knots
+ * at u_k = k/(n-1) (src/spectrum.jl:75-83; 600 knots by default,
+ * src/photon_propagation_setup.jl:18). */
+enum pp_spectrum_mapping {
+    PP_SPEC_KNOTS = 0,     /* x = u*(n-1): exact knots                     */
+    PP_SPEC_TEXTURE = 1    /* x = u*n - 0.5, clamped: 1-D texture addressing */
+};
+
+/* ---- configuration ---------------------------------------------------- */
+enum pp_search_kind { PP_SEARCH_AUTO = 0, PP_SEARCH_BRUTE = 1, PP_SEARCH_GRID = 2 };
+
+typedef struct pp_config {
+    int32_t block_size;         /* threads per block, multiple of 64; 0 = 256 */
+    int32_t waves_per_simd;     /* grid sizing: blocks = CUs*4*wps*64/block; 0 = auto */
+    int32_t search;             /* enum pp_search_kind                         */
+    float grid_cell;            /* grid cell edge (m); 0 = auto                */
+    int32_t deterministic_order;/* pp_copy_hits sorts by (source, photon id)   */
+    int32_t reserved[11];
+} pp_config; /* 64 bytes */
+
+/* ---- random stream layout (the reproducibility contract) ---------------
+ * Every photon owns a Philox4x32-10 stream: key = (seed lo, seed hi),
+ * counter = (word, photon id lo, photon id hi, source index). Words:
+ *   PP_CTR_INIT                  draws at emission: [wavelength, e1, e2, e3]
+ *                                (e1..e3: the emitter's draws in the order of
+ *                                src/photon_prop_cuda.jl:158-276)
+ *   PP_CTR_INIT_EXTRA | 1        Cherenkov direction: [phi, track u1, track u2, -]
+ *   s = 1..n_steps               step s: [step eta, phase branch, phase cos, phi]
+ *   PP_CTR_SAMPLER | sub<<24 | i rejection samplers, block i of sub-stream sub;
+ *                                block PP_SAMPLER_BOOST_BLOCK holds the
+ *                                shape<1 boost uniform.
+ * Uniforms: [0,1) = (x>>8)*2^-24; (0,1] = ((x>>8)+1)*2^-24 where a log follows. */
+#define PP_CTR_INIT 0x00000000u
+#define PP_CTR_INIT_EXTRA 0x40000000u
+#define PP_CTR_SAMPLER 0x80000000u
+#define PP_SAMPLER_BOOST_BLOCK 0x00FFFFFFu
+#define PP_SUB_LONG_GAMMA 0u
+#define PP_SUB_BETA_X 1u
+#define PP_SUB_BETA_Y 2u
+#define PP_SUB_NORMAL 3u
+#define PP_GAMMA_MAX_BLOCKS 64u
+#define PP_MAX_STEPS 0x3FFFFFFF
+
+typedef struct pp_ctx pp_ctx;
+
+/* Version of this ABI (PP_ABI_VERSION). */
+int pp_abi_version(void);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* pp_last_error(void);
+
+/* Number of visible HIP devices (0 if none). */
+int pp_device_count(void);
+
+/* Total and free memory of a device in bytes (sizing of make_hit_buffers'
+ * reserved_memory_fraction, src/photon_prop_cuda.jl:879-880). */
+int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes);
+
+/* make_hit_buffers (src/photon_prop_cuda.jl:878-886): create a context on
+ * `device` with a device hit buffer of `hit_capacity` records. The host
+ * buffer stays with the caller (see pp_copy_hits). cfg may be NULL. */
+int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* cfg);
+int pp_destroy(pp_ctx* ctx);
+
+/* Device memory of one hit record in bytes (capacity sizing). */
+int64_t pp_hit_record_bytes(void);
+
+/* Medium used by the next pp_propagate (by-value kernel argument in the
+ * reference, src/photon_prop_cuda.jl:438). */
+int pp_set_medium(pp_ctx* ctx, const pp_medium* medium);
+
+/* Spectrum: if table == NULL, monochromatic `mono_wl` nm; else n >= 2 knots
+ * of the inverse CDF, mapped with `mapping` (enum pp_spectrum_mapping). */
+int pp_set_spectrum(pp_ctx* ctx, float mono_wl, const float* table, int n, int mapping);
+
+/* Targets uploaded to the device (src/photon_prop_cuda.jl:911). */
+int pp_set_targets(pp_ctx* ctx, const pp_target* targets, int n);
+
+/* run_photon_prop_no_local_cache! (src/photon_prop_cuda.jl:889-959) without
+ * the hit copy-back: propagates every source, each photon from its own
+ * Philox4x32-10 stream (key = seed, counter = (draw, photon id, source)).
+ * Photon count per source: floor(float(photons) * photon_scaling) as in
+ * src/photon_prop_cuda.jl:456. Hits are reset at the start of each call.
+ * Returns PP_ERR_HIT_OVERFLOW if the buffer overflowed; then *n_hits_out is
+ * the capacity and *n_ph_sim_out counts only completed photons. */
+int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t seed,
+                 int32_t n_steps, float photon_scaling, int64_t* n_hits_out,
+                 int64_t* n_ph_sim_out);
+
+/* Copy hits [first, first+n) into caller-owned host SoA columns (the
+ * PhotonHit fields, src/photon_prop_cuda.jl:407-416). Any pointer may be NULL
+ * to skip that column. pos3/dir3/init_dir3 are n x 3 row-major. time is float
+ * if time_f64 == 0, else double. module_ix is the 1-based target index.
+ * photon_id / source_ix identify the emitting photon (deterministic key).
+ * With cfg->deterministic_order the records are ordered by (source, photon). */
+int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3,
+                 float* init_dir3, float* wavelength, void* time, int time_f64,
+                 float* dist_travelled, int64_t* module_ix, int32_t* n_steps,
+                 uint64_t* photon_id, int32_t* source_ix);
+
+/* Device time (ms, HIP events on the context stream) of the propagation
+ * kernels of the last pp_propagate, and the number of kernel launches. */
+int pp_last_timing(pp_ctx* ctx, float* kernel_ms, int32_t* n_launches);
+
+/* The context's hipStream_t, as an opaque pointer. */
+void* pp_stream(pp_ctx* ctx);
+
+/* ---- step-function evaluation (parity tests) ----------------------------
+ * Evaluates one device step function over n explicit inputs on the GPU,
+ * with the same inlined code the propagation kernel uses. `fn` selects the
+ * function; `in` holds n*in_stride floats, `out` n*out_stride floats
+ * (host pointers). Functions and their layouts:
+ *   PP_FN_ISOTROPIC_DIR   in (u1,u2)             out (x,y,z)   jl:73-77
+ *   PP_FN_ROT_FROM_EZ     in (ax,ay,az,bx,by,bz) out (x,y,z)   jl:92-95
+ *   PP_FN_HG_COS          in (g,u)               out (cos)
+ *   PP_FN_ES_COS          in (b,u)               out (cos)
+ *   PP_FN_UPDATE_DIR      in (dx,dy,dz,cos,uphi) out (x,y,z)   jl:280-298
+ *   PP_FN_SPHERE_ISEC     in (cx,cy,cz,r,px,py,pz,dx,dy,dz,step) out (hit,d) jl:309-351
+ *   PP_FN_STEP_LEN        in (u,sca_len)         out (step)    jl:488-490
+ *   PP_FN_CASCADIA        in (wl)                out (sca_len, c_grp, n_phase)
+ *   PP_FN_TRACK_DIR       in (u1,u2)             out (x,y,z)
+ *   PP_FN_PHILOX          in (c0..c3,k0,k1 as u32 bits) out (r0..r3 as u32 bits)
+ *   PP_FN_GAMMA           in (k, theta, key_lo, key_hi, pid_lo, src as u32 bits) out (x)
+ *   PP_FN_SPECTRUM        in (u)                 out (wl)  (uses the context's spectrum)
+ * The medium for PP_FN_CASCADIA is the context's medium. */
+enum pp_fn_id {
+    PP_FN_ISOTROPIC_DIR = 0,
+    PP_FN_ROT_FROM_EZ = 1,
+    PP_FN_HG_COS = 2,
+    PP_FN_ES_COS = 3,
+    PP_FN_UPDATE_DIR = 4,
+    PP_FN_SPHERE_ISEC = 5,
+    PP_FN_STEP_LEN = 6,
+    PP_FN_CASCADIA = 7,
+    PP_FN_TRACK_DIR = 8,
+    PP_FN_PHILOX = 9,
+    PP_FN_GAMMA = 10,
+    PP_FN_SPECTRUM = 11,
+    PP_FN_COUNT = 12
+};
+int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out,
+               int out_stride, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTONPROP_HIP_H */

@@ -1,0 +1,278 @@
+// pp_device.h - device-side step functions of the gfx950 photon propagator.
+//
+// Each function is the device form of one step of the per-photon algorithm of
+// PhotonPropagation.jl's kernel (src/photon_prop_cuda.jl), written as a pure
+// function of explicit uniforms so that it can be evaluated stand-alone
+// (pp_eval_fn) and inside the propagation kernel from the same inlined code.
+// Forms are chosen for the FP32 VALU: reciprocal multiplies instead of
+// divisions where the value is reused, sqrt((1-x)(1+x)) instead of
+// sin(acos(x)), a one-division rotation, a one-cube-root Cardano solution.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/photonprop_hip.h"
+
+namespace pp {
+
+constexpr float kTwoPi = 6.28318548202514648438f;  // float(2*pi), as T(2*pi) in Julia
+constexpr float kCVacMNs = 0.299792458f;          // src/photon_prop_cuda.jl:37
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10. Round: (hi(M1*c2)^c1^k0, lo(M1*c2), hi(M0*c0)^c3^k1, lo(M0*c0));
+// the key is bumped by the Weyl constants between rounds. Each 32x32->64
+// product is one v_mad_u64_u32; the key schedule stays in SGPRs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c.z;
+        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1,
+                       (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// [0,1) and (0,1] with 24 random bits (exact integer->float conversion)
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+__device__ __forceinline__ float u01_open0(uint32_t x) {
+    return (float)((x >> 8) + 1u) * 0x1p-24f;
+}
+
+// Identity of one photon's random stream.
+struct Stream {
+    uint32_t k0, k1;    // seed
+    uint32_t pid_lo, pid_hi;
+    uint32_t src;
+    __device__ __forceinline__ uint4 block(uint32_t word) const {
+        return philox(make_uint4(word, pid_lo, pid_hi, src), k0, k1);
+    }
+};
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 make3(float x, float y, float z) { return f3{x, y, z}; }
+
+// ---------------------------------------------------------------------------
+// Directions
+// ---------------------------------------------------------------------------
+
+// (sin t cos p, sin t sin p, cos t) from cos t and p = 2*pi*u
+__device__ __forceinline__ f3 dir_from_cos_phi(float ct, float st, float u_phi) {
+    float sp, cp;
+    sincosf(u_phi * kTwoPi, &sp, &cp);
+    return make3(cp * st, sp * st, ct);
+}
+
+// initialize_direction_isotropic: cos t = U(-1,1), phi = U(0,2pi)
+//   src/photon_prop_cuda.jl:73-77
+__device__ __forceinline__ f3 isotropic_dir(float u1, float u2) {
+    const float ct = -1.0f + u1 * 2.0f;
+    const float st = sqrtf((1.0f - ct) * (1.0f + ct));
+    return dir_from_cos_phi(ct, st, u2);
+}
+
+// rot_from_ez_fast(a, b): the minimal rotation taking e_z to a, applied to b.
+// With v = e_z x a = (-ay, ax, 0), c = az:  R b = c b + v x b + h v (v.b),
+// h = 1/(1+c) for c >= 0 and (1-c)/|v|^2 for c < 0 (same value for unit a,
+// no cancellation on either side). a = -e_z exactly turns by pi about x.
+__device__ __forceinline__ f3 rot_from_ez(f3 a, f3 b) {
+    const float c = a.z;
+    const float s2 = a.x * a.x + a.y * a.y;
+    const bool up = c >= 0.0f;
+    const float h = (up ? 1.0f : (1.0f - c)) / (up ? (1.0f + c) : s2);
+    const float vdb = a.x * b.y - a.y * b.x;
+    f3 o;
+    o.x = c * b.x + a.x * b.z - h * a.y * vdb;
+    o.y = c * b.y + a.y * b.z + h * a.x * vdb;
+    o.z = c * b.z - (a.x * b.x + a.y * b.y);
+    if (!up && s2 == 0.0f) o = make3(b.x, -b.y, -b.z);
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Phase functions (CherenkovMediumBase samplers; inverse CDFs)
+// ---------------------------------------------------------------------------
+struct Phase {
+    int kind;          // pp_phase_kind
+    float hg_fraction;
+    // HG: cos = A - C t^2, t = B / (D0 + D1 u)
+    float hg_A, hg_B, hg_C, hg_D0, hg_D1;
+    // ES: q = Q (1 - 2u); x = sgn (W - P3/W), W = cbrt(|q|/2 + sqrt(q^2/4 + P3^3))
+    float es_Q, es_P3, es_P3cube;
+};
+
+__host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg_fraction) {
+    Phase p;
+    p.kind = kind;
+    p.hg_fraction = hg_fraction;
+    p.hg_A = (1.0f + g * g) / (2.0f * g);
+    p.hg_B = 1.0f - g * g;
+    p.hg_C = 1.0f / (2.0f * g);
+    p.hg_D0 = 1.0f - g;
+    p.hg_D1 = 2.0f * g;
+    p.es_Q = (3.0f + b) / b;
+    p.es_P3 = 1.0f / b;
+    p.es_P3cube = p.es_P3 * p.es_P3 * p.es_P3;
+    return p;
+}
+
+__device__ __forceinline__ float clamp1(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
+
+__device__ __forceinline__ float hg_cos(const Phase& p, float u) {
+    const float t = p.hg_B / (p.hg_D0 + p.hg_D1 * u);
+    return clamp1(p.hg_A - p.hg_C * (t * t));
+}
+
+__device__ __forceinline__ float es_cos(const Phase& p, float u) {
+    const float q = p.es_Q * (1.0f - 2.0f * u);
+    const float hq = 0.5f * fabsf(q);
+    const float w = cbrtf(hq + sqrtf(hq * hq + p.es_P3cube));
+    const float x = w - p.es_P3 / w;
+    return clamp1(q > 0.0f ? -x : x);
+}
+
+__device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float u) {
+    if (p.kind == PP_PHASE_HG) return hg_cos(p, u);
+    return u_choice < p.hg_fraction ? hg_cos(p, u) : es_cos(p, u);
+}
+
+// update_direction   src/photon_prop_cuda.jl:280-298
+__device__ __forceinline__ f3 update_direction(f3 dir, float ct, float u_phi) {
+    const float st = sqrtf(1.0f - ct * ct);
+    return rot_from_ez(dir, dir_from_cos_phi(ct, st, u_phi));
+}
+
+// ---------------------------------------------------------------------------
+// Ray-sphere test: _intersect_nsphere   src/photon_prop_cuda.jl:309-351
+// Returns the near root d if 0 < d < step, else a value that fails the test.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool sphere_hit(float4 t, f3 pos, f3 dir, float step, float& d) {
+    const float dx = pos.x - t.x, dy = pos.y - t.y, dz = pos.z - t.z;
+    const float a = dir.x * dx + dir.y * dy + dir.z * dz;
+    const float pp = dx * dx + dy * dy + dz * dz;
+    const float b = a * a - (pp - t.w);  // t.w = r^2
+    d = (b >= 0.0f) ? (-a - sqrtf(fabsf(b))) : -1.0f;
+    return (d > 0.0f) & (d < step);
+}
+
+// ---------------------------------------------------------------------------
+// Medium: per-wavelength values   src/photon_prop_cuda.jl:480-483
+// ---------------------------------------------------------------------------
+struct Medium {
+    int kind;  // pp_medium_kind
+    Phase phase;
+    // SIMPLE
+    float sca_len, c_grp, n_phase;
+    // CASCADIA: Kopelevich b = k1 x^-4.3 + k2 x^-1.7 + k3 x^-0.3, x = wl/550
+    float kop1, kop2, kop3, sca_scale;
+    // Quan-Fry: n = a0 + a1/wl + a2/wl^2 + a3/wl^3 ; n_g = a0 + 2a1/wl + 3a2/wl^2 + 4a3/wl^3
+    float qf0, qf1, qf2, qf3;
+};
+
+struct MediumValues {
+    float sca_len, inv_c_grp, cos_c, sin_c;
+};
+
+__device__ __forceinline__ MediumValues medium_values(const Medium& m, float wl) {
+    MediumValues v;
+    float n;
+    if (m.kind == PP_MEDIUM_SIMPLE) {
+        v.sca_len = m.sca_len;
+        v.inv_c_grp = 1.0f / m.c_grp;
+        n = m.n_phase;
+    } else {
+        const float lx = log2f(wl * (1.0f / 550.0f));
+        const float b = m.kop1 * exp2f(-4.3f * lx) + m.kop2 * exp2f(-1.7f * lx) +
+                        m.kop3 * exp2f(-0.3f * lx);
+        v.sca_len = m.sca_scale / b;
+        const float il = 1.0f / wl;
+        n = m.qf0 + il * (m.qf1 + il * (m.qf2 + il * m.qf3));
+        const float ng = m.qf0 + il * (2.0f * m.qf1 + il * (3.0f * m.qf2 + il * (4.0f * m.qf3)));
+        v.inv_c_grp = ng * (1.0f / kCVacMNs);
+    }
+    v.cos_c = 1.0f / n;  // cherenkov_angle: acos(1/n), beta = 1
+    v.sin_c = sqrtf((1.0f - v.cos_c) * (1.0f + v.cos_c));
+    return v;
+}
+
+// sample_cherenkov_track_direction: 1 - cos t follows x^(a-1) exp(-b x^a),
+// a = 0.39, b = 2.61, truncated at x = 2 (inverse CDF); phi uniform.
+__device__ __forceinline__ f3 cherenkov_track_dir(float u1, float u2) {
+    constexpr float kInvA = 1.0f / 0.39f;
+    constexpr float kInvB = 1.0f / 2.61f;
+    constexpr float kI = 0.9672917f;  // 1 - exp(-2.61 * 2^0.39)
+    const float t = -logf(1.0f - u1 * kI) * kInvB;
+    const float ct = fmaxf(1.0f - powf(t, kInvA), -1.0f);
+    const float st = sqrtf((1.0f - ct) * (1.0f + ct));
+    return dir_from_cos_phi(ct, st, u2);
+}
+
+// ---------------------------------------------------------------------------
+// Samplers on the photon's sampler sub-streams
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t sampler_word(uint32_t sub, uint32_t i) {
+    return PP_CTR_SAMPLER | (sub << 24) | i;
+}
+
+__device__ __forceinline__ float normal_sample(const Stream& s, uint32_t sub) {
+    const uint4 r = s.block(sampler_word(sub, 0));
+    float sp, cp;
+    sincosf(u01(r.y) * kTwoPi, &sp, &cp);
+    return sqrtf(-2.0f * logf(u01_open0(r.x))) * cp;
+}
+
+// Marsaglia-Tsang gamma(shape, scale). Each sampler block: a Box-Muller pair
+// and two acceptance uniforms (two attempts). shape < 1 boosts Gamma(shape+1)
+// by U^(1/shape), U from the sub-stream's boost block.
+__device__ inline float gamma_sample(float shape, float scale, const Stream& s, uint32_t sub) {
+    float boost = 1.0f;
+    if (shape < 1.0f) {
+        const uint4 r = s.block(sampler_word(sub, PP_SAMPLER_BOOST_BLOCK));
+        boost = powf(u01_open0(r.x), 1.0f / shape);
+        shape += 1.0f;
+    }
+    const float d = shape - 1.0f / 3.0f;
+    const float c = 1.0f / sqrtf(9.0f * d);
+    for (uint32_t i = 0; i < PP_GAMMA_MAX_BLOCKS; ++i) {
+        const uint4 r = s.block(sampler_word(sub, i));
+        const float rad = sqrtf(-2.0f * logf(u01_open0(r.x)));
+        float sp, cp;
+        sincosf(u01(r.y) * kTwoPi, &sp, &cp);
+        const float z[2] = {rad * cp, rad * sp};
+        const float ua[2] = {u01_open0(r.z), u01_open0(r.w)};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const float x = z[k];
+            float v = 1.0f + c * x;
+            if (v <= 0.0f) continue;
+            v = v * v * v;
+            const float u = ua[k];
+            const float x2 = x * x;
+            if (u < 1.0f - 0.0331f * (x2 * x2) || logf(u) < 0.5f * x2 + d * (1.0f - v + logf(v)))
+                return d * v * scale * boost;
+        }
+    }
+    return d * scale * boost;
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum: inverse-CDF knots, linear interpolation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float spectrum_sample(const float* tab, int n, int mapping, float mono,
+                                                 float u) {
+    if (n == 0) return mono;
+    float x = mapping == PP_SPEC_TEXTURE ? u * (float)n - 0.5f : u * (float)(n - 1);
+    x = fminf(fmaxf(x, 0.0f), (float)(n - 1));
+    const int i = min((int)x, n - 2);
+    const float f = x - (float)i;
+    const float a = tab[i], b = tab[i + 1];
+    return a + f * (b - a);
+}
+
+}  // namespace pp

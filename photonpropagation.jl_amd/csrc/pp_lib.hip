@@ -1,0 +1,862 @@
+// pp_lib.hip - libphotonprop_hip.so: gfx950 photon propagation kernels and the
+// C ABI declared in include/photonprop_hip.h.
+//
+// Kernel design (one photon per lane, photons grid-strided over the launch):
+//   * targets (float4 x,y,z,r^2) and the spectrum's inverse-CDF knots are
+//     staged in LDS once per workgroup; target reads are wave-uniform
+//     broadcasts;
+//   * every photon draws from its own Philox4x32-10 stream (counter =
+//     (word, photon id, source)), so hits do not depend on the launch shape;
+//   * one Philox block per step supplies all four uniforms of a step;
+//   * hits are compacted per wave: ballot + mbcnt rank, one 64-bit atomic per
+//     wave, coalesced SoA stores; slots past capacity are dropped and flagged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pp_device.h"
+
+using pp::f3;
+
+// ---------------------------------------------------------------------------
+// device data
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNumFloatCols = 12;  // pos xyz, dir xyz, init dir xyz, wl, time, dist
+
+struct HitCols {
+    float* f[kNumFloatCols];
+    int32_t* module_ix;  // 1-based target index
+    int32_t* n_steps;
+    uint64_t* photon_id;
+    int32_t* source_ix;
+    uint64_t capacity;
+};
+
+struct Counters {
+    unsigned long long n_hits;    // reserved slots (may exceed capacity)
+    unsigned long long n_ph_sim;  // completed photons
+    unsigned int err;             // 1 = overflow
+    unsigned int pad;
+};
+
+struct KParams {
+    pp_source src;
+    pp::Medium med;
+    uint32_t k0, k1, src_ix;
+    int32_t n_steps;
+    uint64_t n_photons;
+    const float4* targets;
+    int32_t n_targets;
+    int32_t spec_n;
+    int32_t spec_map;
+    float spec_mono;
+    const float* spec;
+    HitCols hits;
+    Counters* counters;
+};
+
+// ---------------------------------------------------------------------------
+// photon emission   src/photon_prop_cuda.jl:158-276
+// ---------------------------------------------------------------------------
+struct PhotonInit {
+    f3 pos, dir;
+    float time, wl;
+    pp::MediumValues mv;
+};
+
+__device__ __forceinline__ f3 add_scaled(f3 p, float s, f3 d) {
+    return pp::make3(p.x + s * d.x, p.y + s * d.y, p.z + s * d.z);
+}
+
+// sample_cherenkov_direction[_no_smear|_custom_smear]  :80-111, :134-154
+__device__ __forceinline__ f3 cherenkov_dir(const pp_source& src, const pp::MediumValues& mv,
+                                            const pp::Stream& s, int smear) {
+    const uint4 x = s.block(PP_CTR_INIT_EXTRA | 1u);
+    f3 ph = pp::dir_from_cos_phi(mv.cos_c, mv.sin_c, pp::u01(x.x));
+    if (smear == 1) {
+        ph = pp::rot_from_ez(pp::cherenkov_track_dir(pp::u01(x.y), pp::u01(x.z)), ph);
+    } else if (smear == 2) {
+        const float gx = pp::gamma_sample(src.beta_a, 1.0f, s, PP_SUB_BETA_X);
+        const float gy = pp::gamma_sample(src.beta_b, 1.0f, s, PP_SUB_BETA_Y);
+        const float tc = (gx / (gx + gy)) * 2.0f - 1.0f;
+        const float ts = sqrtf((1.0f - tc) * (1.0f + tc));
+        ph = pp::rot_from_ez(pp::dir_from_cos_phi(tc, ts, pp::u01(x.y)), ph);
+    }
+    const f3 axis = pp::make3(src.direction[0], src.direction[1], src.direction[2]);
+    return pp::rot_from_ez(axis, ph);
+}
+
+__device__ __forceinline__ PhotonInit init_photon(const KParams& p, const float* spec,
+                                                  const pp::Stream& s) {
+    const pp_source& src = p.src;
+    const uint4 r = s.block(PP_CTR_INIT);
+    PhotonInit st;
+    st.wl = pp::spectrum_sample(spec, p.spec_n, p.spec_map, p.spec_mono, pp::u01(r.x));
+    st.mv = pp::medium_values(p.med, st.wl);
+    st.pos = pp::make3(src.position[0], src.position[1], src.position[2]);
+    st.time = src.time;
+    const f3 axis = pp::make3(src.direction[0], src.direction[1], src.direction[2]);
+    switch (src.kind) {
+    case PP_SRC_POINTLIKE_ISOTROPIC:
+        st.dir = pp::isotropic_dir(pp::u01(r.y), pp::u01(r.z));
+        break;
+    case PP_SRC_POINTLIKE_TIME_RANGE:
+        st.dir = pp::isotropic_dir(pp::u01(r.y), pp::u01(r.z));
+        st.time = src.time + pp::u01(r.w) * (src.time_end - src.time);
+        break;
+    case PP_SRC_AXICONE: {
+        float theta = src.angle;
+        if (src.beam_divergence > 0.0f)
+            theta += pp::normal_sample(s, PP_SUB_NORMAL) * src.beam_divergence;
+        float sn, cs;
+        sincosf(theta, &sn, &cs);
+        st.dir = pp::rot_from_ez(axis, pp::dir_from_cos_phi(cs, sn, pp::u01(r.y)));
+        break;
+    }
+    case PP_SRC_COLLIMATED_ISOTROPIC: {
+        const float ct = src.cut_angle_cos + pp::u01(r.z) * (1.0f - src.cut_angle_cos);
+        st.dir = pp::dir_from_cos_phi(ct, sqrtf((1.0f - ct) * (1.0f + ct)), pp::u01(r.y));
+        break;
+    }
+    case PP_SRC_PENCIL:
+        if (src.beam_divergence > 0.0f) {
+            const float theta =
+                sqrtf(pp::u01(r.y) * (src.beam_divergence * src.beam_divergence));
+            float sn, cs;
+            sincosf(theta, &sn, &cs);
+            st.dir = pp::rot_from_ez(axis, pp::dir_from_cos_phi(cs, sn, pp::u01(r.z)));
+        } else {
+            st.dir = axis;
+        }
+        break;
+    case PP_SRC_EXTENDED_CHERENKOV:
+    case PP_SRC_EXTENDED_CHERENKOV_NO_SMEAR:
+    case PP_SRC_EXTENDED_CHERENKOV_CUSTOM_SMEAR: {
+        const float lp =
+            pp::gamma_sample(src.long_a, 1.0f / src.long_b, s, PP_SUB_LONG_GAMMA) * src.long_lrad;
+        st.pos = add_scaled(st.pos, lp, axis);
+        st.time = src.time + lp * (1.0f / pp::kCVacMNs);
+        const int smear = src.kind == PP_SRC_EXTENDED_CHERENKOV ? 1
+                          : src.kind == PP_SRC_EXTENDED_CHERENKOV_NO_SMEAR ? 0 : 2;
+        st.dir = cherenkov_dir(src, st.mv, s, smear);
+        break;
+    }
+    case PP_SRC_POINTLIKE_CHERENKOV:
+        st.dir = cherenkov_dir(src, st.mv, s, 1);
+        break;
+    case PP_SRC_CHERENKOV_TRACK:
+    default: {
+        const float along = pp::u01(r.y) * src.length;
+        st.pos = add_scaled(st.pos, along, axis);
+        st.time = src.time + along * (1.0f / pp::kCVacMNs);
+        st.dir = cherenkov_dir(src, st.mv, s, 1);
+        break;
+    }
+    }
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// wave-compacted hit store
+// ---------------------------------------------------------------------------
+struct HitRec {
+    f3 pos, dir, init_dir;
+    float wl, time, dist;
+    int32_t module_ix, n_steps;
+    uint64_t pid;
+};
+
+// Returns false if this lane's record did not fit (overflow).
+__device__ __forceinline__ bool store_hits(bool has_hit, const HitRec& h, uint32_t src_ix,
+                                           const HitCols& out, Counters* ctr) {
+    const uint64_t mask = __ballot(has_hit);
+    if (mask == 0) return true;
+    const uint32_t lane = __lane_id();
+    const uint32_t rank =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&ctr->n_hits, (unsigned long long)__popcll(mask));
+    const uint32_t blo = __builtin_amdgcn_readlane((uint32_t)base, leader);
+    const uint32_t bhi = __builtin_amdgcn_readlane((uint32_t)(base >> 32), leader);
+    base = ((unsigned long long)bhi << 32) | blo;
+    if (!has_hit) return true;
+    const uint64_t i = base + rank;
+    if (i >= out.capacity) return false;
+    out.f[0][i] = h.pos.x;
+    out.f[1][i] = h.pos.y;
+    out.f[2][i] = h.pos.z;
+    out.f[3][i] = h.dir.x;
+    out.f[4][i] = h.dir.y;
+    out.f[5][i] = h.dir.z;
+    out.f[6][i] = h.init_dir.x;
+    out.f[7][i] = h.init_dir.y;
+    out.f[8][i] = h.init_dir.z;
+    out.f[9][i] = h.wl;
+    out.f[10][i] = h.time;
+    out.f[11][i] = h.dist;
+    out.module_ix[i] = h.module_ix;
+    out.n_steps[i] = h.n_steps;
+    out.photon_id[i] = h.pid;
+    out.source_ix[i] = (int32_t)src_ix;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// propagation kernel: cuda_propagate_photons!   src/photon_prop_cuda.jl:429-549
+// ---------------------------------------------------------------------------
+template <int PHASE>
+__global__ void __launch_bounds__(1024) propagate_brute(KParams p) {
+    extern __shared__ float4 lds[];
+    float4* s_tgt = lds;
+    float* s_spec = reinterpret_cast<float*>(lds + p.n_targets);
+    for (int i = threadIdx.x; i < p.n_targets; i += blockDim.x) s_tgt[i] = p.targets[i];
+    for (int i = threadIdx.x; i < p.spec_n; i += blockDim.x) s_spec[i] = p.spec[i];
+    __syncthreads();
+
+    pp::Medium med = p.med;
+    med.phase.kind = PHASE;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long n_done = 0;
+    const int nt = p.n_targets;
+
+    for (; pid < p.n_photons; pid += stride) {
+        pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), p.src_ix};
+        const PhotonInit st = init_photon(p, s_spec, s);
+        f3 pos = st.pos, dir = st.dir;
+        float time = st.time, dist = 0.0f;
+        const float sca_len = st.mv.sca_len, inv_c = st.mv.inv_c_grp;
+        bool hit = false;
+        int32_t module_ix = 0, n_at = 0;
+
+        for (int32_t step = 1; step <= p.n_steps; ++step) {
+            const uint4 r = s.block((uint32_t)step);
+            const float step_size = -logf(pp::u01_open0(r.x)) * sca_len;
+            // first target in index order whose near root lies inside the step
+            int found = -1;
+            float dt = 0.0f;
+            for (int t = 0; t < nt; ++t) {
+                float d;
+                if (pp::sphere_hit(s_tgt[t], pos, dir, step_size, d) && found < 0) {
+                    found = t;
+                    dt = d;
+                }
+            }
+            if (found >= 0) {
+                pos = add_scaled(pos, dt, dir);
+                dist += dt;
+                time += dt * inv_c;
+                hit = true;
+                module_ix = found + 1;
+                n_at = step;
+                break;
+            }
+            pos = add_scaled(pos, step_size, dir);
+            dist += step_size;
+            time += step_size * inv_c;
+            const float ct = pp::phase_cos(med.phase, pp::u01(r.y), pp::u01(r.z));
+            dir = pp::update_direction(dir, ct, pp::u01(r.w));
+        }
+
+        HitRec h{pos, dir, st.dir, st.wl, time, dist, module_ix, n_at, pid};
+        if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
+            atomicOr(&p.counters->err, 1u);
+            break;
+        }
+        ++n_done;
+    }
+    if (n_done) atomicAdd(&p.counters->n_ph_sim, n_done);
+}
+
+// ---------------------------------------------------------------------------
+// step-function evaluation kernel (parity tests)
+// ---------------------------------------------------------------------------
+struct EvalParams {
+    int fn;
+    int in_stride, out_stride;
+    int64_t n;
+    const float* in;
+    float* out;
+    pp::Medium med;
+    const float* spec;
+    int spec_n, spec_map;
+    float spec_mono;
+};
+
+__device__ __forceinline__ uint32_t bits(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ float as_f(uint32_t x) { return __uint_as_float(x); }
+
+__global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e.n) return;
+    const float* x = e.in + i * e.in_stride;
+    float* y = e.out + i * e.out_stride;
+    switch (e.fn) {
+    case PP_FN_ISOTROPIC_DIR: {
+        f3 d = pp::isotropic_dir(x[0], x[1]);
+        y[0] = d.x; y[1] = d.y; y[2] = d.z;
+        break;
+    }
+    case PP_FN_ROT_FROM_EZ: {
+        f3 d = pp::rot_from_ez(pp::make3(x[0], x[1], x[2]), pp::make3(x[3], x[4], x[5]));
+        y[0] = d.x; y[1] = d.y; y[2] = d.z;
+        break;
+    }
+    case PP_FN_HG_COS: {
+        pp::Phase ph = pp::make_phase(PP_PHASE_HG, x[0], 1.0f, 1.0f);
+        y[0] = pp::hg_cos(ph, x[1]);
+        break;
+    }
+    case PP_FN_ES_COS: {
+        pp::Phase ph = pp::make_phase(PP_PHASE_MIXED_HG_ES, 0.5f, x[0], 0.0f);
+        y[0] = pp::es_cos(ph, x[1]);
+        break;
+    }
+    case PP_FN_UPDATE_DIR: {
+        f3 d = pp::update_direction(pp::make3(x[0], x[1], x[2]), x[3], x[4]);
+        y[0] = d.x; y[1] = d.y; y[2] = d.z;
+        break;
+    }
+    case PP_FN_SPHERE_ISEC: {
+        float d;
+        const bool h = pp::sphere_hit(make_float4(x[0], x[1], x[2], x[3] * x[3]),
+                                      pp::make3(x[4], x[5], x[6]), pp::make3(x[7], x[8], x[9]),
+                                      x[10], d);
+        y[0] = h ? 1.0f : 0.0f;
+        y[1] = h ? d : 0.0f;
+        break;
+    }
+    case PP_FN_STEP_LEN:
+        y[0] = -logf(x[0]) * x[1];
+        break;
+    case PP_FN_CASCADIA: {
+        pp::MediumValues mv = pp::medium_values(e.med, x[0]);
+        y[0] = mv.sca_len;
+        y[1] = 1.0f / mv.inv_c_grp;
+        y[2] = 1.0f / mv.cos_c;
+        break;
+    }
+    case PP_FN_TRACK_DIR: {
+        f3 d = pp::cherenkov_track_dir(x[0], x[1]);
+        y[0] = d.x; y[1] = d.y; y[2] = d.z;
+        break;
+    }
+    case PP_FN_PHILOX: {
+        uint4 r = pp::philox(make_uint4(bits(x[0]), bits(x[1]), bits(x[2]), bits(x[3])),
+                             bits(x[4]), bits(x[5]));
+        y[0] = as_f(r.x); y[1] = as_f(r.y); y[2] = as_f(r.z); y[3] = as_f(r.w);
+        break;
+    }
+    case PP_FN_GAMMA: {
+        pp::Stream s{bits(x[2]), bits(x[3]), bits(x[4]), bits(x[5]), bits(x[6])};
+        y[0] = pp::gamma_sample(x[0], x[1], s, PP_SUB_LONG_GAMMA);
+        break;
+    }
+    case PP_FN_SPECTRUM:
+        y[0] = pp::spectrum_sample(e.spec, e.spec_n, e.spec_map, e.spec_mono, x[0]);
+        break;
+    default:
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PP_HIP(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(PP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+pp::Medium lower_medium(const pp_medium& m) {
+    pp::Medium d{};
+    d.kind = m.kind;
+    d.phase = pp::make_phase(m.phase_kind, m.hg_g, m.es_b, m.hg_fraction);
+    d.sca_len = m.sca_len;
+    d.c_grp = m.group_velocity;
+    d.n_phase = m.phase_refractive_index;
+    d.sca_scale = m.sca_scale;
+    d.kop1 = 0.0017f;
+    d.kop2 = (float)(1.34 * (double)m.vol_conc_small);
+    d.kop3 = (float)(0.312 * (double)m.vol_conc_large);
+    // Quan & Fry (1995) coefficients; linear pressure term 1.45e-5 / atm
+    const double S = m.salinity, T = m.temperature, P = m.pressure;
+    d.qf0 = (float)(1.31405 + (1.779e-4 - 1.05e-6 * T + 1.6e-8 * T * T) * S - 2.02e-6 * T * T +
+                    1.45e-5 * P);
+    d.qf1 = (float)(15.868 + 0.01155 * S - 0.00423 * T);
+    d.qf2 = (float)(-4382.0);
+    d.qf3 = (float)(1.1455e6);
+    return d;
+}
+
+bool finite3(const float* v) {
+    return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+}  // namespace
+
+struct pp_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pp_config cfg{};
+    void* hit_mem = nullptr;
+    HitCols hits{};
+    Counters* d_counters = nullptr;
+    bool has_medium = false;
+    pp_medium medium_in{};
+    pp::Medium medium{};
+    bool has_spectrum = false;
+    float spec_mono = 0.0f;
+    int spec_n = 0, spec_map = 0;
+    float* d_spec = nullptr;
+    int spec_cap = 0;
+    int n_targets = -1;
+    float4* d_targets = nullptr;
+    int tgt_cap = 0;
+    int64_t n_hits = 0, n_ph_sim = 0;
+    float last_ms = 0.0f;
+    int32_t n_launch = 0;
+    bool perm_valid = false;
+    std::vector<int64_t> perm;
+};
+
+namespace {
+
+int set_dev(pp_ctx* c) {
+    PP_HIP(hipSetDevice(c->device));
+    return PP_OK;
+}
+
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+}  // namespace
+
+extern "C" {
+
+int pp_abi_version(void) { return PP_ABI_VERSION; }
+
+const char* pp_last_error(void) { return g_err.c_str(); }
+
+int pp_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes) {
+    int ndev = pp_device_count();
+    if (device < 0 || device >= ndev) return fail(PP_ERR_INVALID, "pp_device_memory: no such device");
+    PP_HIP(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    PP_HIP(hipMemGetInfo(&fr, &tot));
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    return PP_OK;
+}
+
+int64_t pp_hit_record_bytes(void) {
+    return kNumFloatCols * 4 + 4 + 4 + 8 + 4;
+}
+
+int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* cfg) {
+    if (!out) return fail(PP_ERR_INVALID, "pp_create: out is NULL");
+    *out = nullptr;
+    if (hit_capacity <= 0) return fail(PP_ERR_INVALID, "pp_create: hit_capacity must be > 0");
+    int ndev = pp_device_count();
+    if (device < 0 || device >= ndev)
+        return fail(PP_ERR_INVALID, "pp_create: no HIP device " + std::to_string(device) +
+                                        " (" + std::to_string(ndev) + " visible)");
+    pp_config c{};
+    if (cfg) c = *cfg;
+    if (c.block_size == 0) c.block_size = 256;
+    if (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64)
+        return fail(PP_ERR_INVALID, "pp_create: block_size must be a multiple of 64 in [64,1024]");
+    if (c.waves_per_simd < 0 || c.waves_per_simd > 32)
+        return fail(PP_ERR_INVALID, "pp_create: waves_per_simd must be in [0,32]");
+    auto* ctx = new pp_ctx();
+    ctx->device = device;
+    ctx->cfg = c;
+    auto cleanup = [&](int code) {
+        pp_destroy(ctx);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(fail(PP_ERR_HIP, "hipSetDevice failed"));
+    if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+        hipSuccess)
+        return cleanup(fail(PP_ERR_HIP, "hipDeviceGetAttribute failed"));
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
+        return cleanup(fail(PP_ERR_HIP, "stream/event creation failed"));
+    const size_t cap = (size_t)hit_capacity;
+    const size_t bytes = cap * (size_t)pp_hit_record_bytes() + 256 * 8;
+    if (hipMalloc(&ctx->hit_mem, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return cleanup(fail(PP_ERR_NOMEM, "pp_create: cannot allocate " + std::to_string(bytes) +
+                                              " bytes of hit buffer"));
+    }
+    char* base = (char*)ctx->hit_mem;
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        char* p = base + off;
+        off += (n + 255) & ~(size_t)255;
+        return p;
+    };
+    for (int i = 0; i < kNumFloatCols; ++i) ctx->hits.f[i] = (float*)take(cap * 4);
+    ctx->hits.module_ix = (int32_t*)take(cap * 4);
+    ctx->hits.n_steps = (int32_t*)take(cap * 4);
+    ctx->hits.photon_id = (uint64_t*)take(cap * 8);
+    ctx->hits.source_ix = (int32_t*)take(cap * 4);
+    ctx->hits.capacity = cap;
+    if (off > bytes + 255 * 20) return cleanup(fail(PP_ERR_STATE, "internal layout error"));
+    if (hipMalloc(&ctx->d_counters, sizeof(Counters)) != hipSuccess)
+        return cleanup(fail(PP_ERR_NOMEM, "pp_create: counters"));
+    *out = ctx;
+    return PP_OK;
+}
+
+int pp_destroy(pp_ctx* ctx) {
+    if (!ctx) return PP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->hit_mem) (void)hipFree(ctx->hit_mem);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_spec) (void)hipFree(ctx->d_spec);
+    if (ctx->d_targets) (void)hipFree(ctx->d_targets);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PP_OK;
+}
+
+int pp_set_medium(pp_ctx* ctx, const pp_medium* m) {
+    if (!ctx || !m) return fail(PP_ERR_INVALID, "pp_set_medium: NULL argument");
+    if (m->kind != PP_MEDIUM_SIMPLE && m->kind != PP_MEDIUM_CASCADIA)
+        return fail(PP_ERR_INVALID, "pp_set_medium: unknown medium kind");
+    if (m->phase_kind != PP_PHASE_HG && m->phase_kind != PP_PHASE_MIXED_HG_ES)
+        return fail(PP_ERR_INVALID, "pp_set_medium: unknown phase function kind");
+    if (!(m->hg_g > 0.0f && m->hg_g < 1.0f))
+        return fail(PP_ERR_INVALID, "pp_set_medium: hg_g must be in (0,1)");
+    if (m->phase_kind == PP_PHASE_MIXED_HG_ES &&
+        !(m->es_b > 0.0f && m->hg_fraction >= 0.0f && m->hg_fraction <= 1.0f))
+        return fail(PP_ERR_INVALID, "pp_set_medium: es_b > 0 and hg_fraction in [0,1] required");
+    if (m->kind == PP_MEDIUM_SIMPLE &&
+        !(m->sca_len > 0.0f && m->group_velocity > 0.0f && m->phase_refractive_index >= 1.0f))
+        return fail(PP_ERR_INVALID, "pp_set_medium: SIMPLE needs sca_len>0, group_velocity>0, n>=1");
+    if (m->kind == PP_MEDIUM_CASCADIA && !(m->sca_scale > 0.0f))
+        return fail(PP_ERR_INVALID, "pp_set_medium: sca_scale must be > 0");
+    ctx->medium_in = *m;
+    ctx->medium = lower_medium(*m);
+    ctx->has_medium = true;
+    return PP_OK;
+}
+
+int pp_set_spectrum(pp_ctx* ctx, float mono_wl, const float* table, int n, int mapping) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_set_spectrum: NULL ctx");
+    if (int rc = set_dev(ctx)) return rc;
+    if (!table) {
+        if (!(mono_wl > 0.0f) || !std::isfinite(mono_wl))
+            return fail(PP_ERR_INVALID, "pp_set_spectrum: monochromatic wavelength must be > 0");
+        ctx->spec_mono = mono_wl;
+        ctx->spec_n = 0;
+        ctx->has_spectrum = true;
+        return PP_OK;
+    }
+    if (n < 2 || n > 8192) return fail(PP_ERR_INVALID, "pp_set_spectrum: n must be in [2, 8192]");
+    if (mapping != PP_SPEC_KNOTS && mapping != PP_SPEC_TEXTURE)
+        return fail(PP_ERR_INVALID, "pp_set_spectrum: unknown mapping");
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(table[i]) || table[i] <= 0.0f)
+            return fail(PP_ERR_INVALID, "pp_set_spectrum: knots must be finite and > 0");
+    if (n > ctx->spec_cap) {
+        if (ctx->d_spec) PP_HIP(hipFree(ctx->d_spec));
+        ctx->d_spec = nullptr;
+        PP_HIP(hipMalloc(&ctx->d_spec, (size_t)n * sizeof(float)));
+        ctx->spec_cap = n;
+    }
+    PP_HIP(hipMemcpy(ctx->d_spec, table, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    ctx->spec_n = n;
+    ctx->spec_map = mapping;
+    ctx->has_spectrum = true;
+    return PP_OK;
+}
+
+int pp_set_targets(pp_ctx* ctx, const pp_target* t, int n) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_set_targets: NULL ctx");
+    if (n < 0 || (n > 0 && !t)) return fail(PP_ERR_INVALID, "pp_set_targets: bad target list");
+    if ((size_t)n * sizeof(float4) + 8192 * sizeof(float) > (size_t)kMaxLdsBytes)
+        return fail(PP_ERR_INVALID, "pp_set_targets: too many targets for LDS staging");
+    if (int rc = set_dev(ctx)) return rc;
+    std::vector<float4> h((size_t)std::max(n, 1));
+    for (int i = 0; i < n; ++i) {
+        if (t[i].shape_kind != PP_SHAPE_SPHERE)
+            return fail(PP_ERR_INVALID, "pp_set_targets: only spherical targets are supported");
+        if (!finite3(t[i].position) || !(t[i].radius > 0.0f) || !std::isfinite(t[i].radius))
+            return fail(PP_ERR_INVALID, "pp_set_targets: target " + std::to_string(i) +
+                                            " has a non-finite position or radius <= 0");
+        h[i] = make_float4(t[i].position[0], t[i].position[1], t[i].position[2],
+                           t[i].radius * t[i].radius);
+    }
+    if (n > ctx->tgt_cap) {
+        if (ctx->d_targets) PP_HIP(hipFree(ctx->d_targets));
+        ctx->d_targets = nullptr;
+        PP_HIP(hipMalloc(&ctx->d_targets, (size_t)n * sizeof(float4)));
+        ctx->tgt_cap = n;
+    }
+    if (n > 0)
+        PP_HIP(hipMemcpy(ctx->d_targets, h.data(), (size_t)n * sizeof(float4),
+                         hipMemcpyHostToDevice));
+    ctx->n_targets = n;
+    return PP_OK;
+}
+
+int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t seed,
+                 int32_t n_steps, float photon_scaling, int64_t* n_hits_out,
+                 int64_t* n_ph_sim_out) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_propagate: NULL ctx");
+    if (n_sources < 0 || (n_sources > 0 && !sources))
+        return fail(PP_ERR_INVALID, "pp_propagate: bad source list");
+    if (!ctx->has_medium || !ctx->has_spectrum || ctx->n_targets < 0)
+        return fail(PP_ERR_STATE, "pp_propagate: set medium, spectrum and targets first");
+    if (n_steps < 0 || n_steps > PP_MAX_STEPS)
+        return fail(PP_ERR_INVALID, "pp_propagate: n_steps out of range");
+    if (!(photon_scaling >= 0.0f) || !std::isfinite(photon_scaling))
+        return fail(PP_ERR_INVALID, "pp_propagate: photon_scaling must be finite and >= 0");
+    for (int i = 0; i < n_sources; ++i) {
+        const pp_source& s = sources[i];
+        if (s.kind < 0 || s.kind >= PP_SRC_KIND_COUNT)
+            return fail(PP_ERR_INVALID, "pp_propagate: source " + std::to_string(i) +
+                                            " has an unknown kind");
+        if (s.photons < 0)
+            return fail(PP_ERR_INVALID, "pp_propagate: negative photon count");
+        if (!finite3(s.position) || !finite3(s.direction))
+            return fail(PP_ERR_INVALID, "pp_propagate: non-finite source position/direction");
+        const bool cascade = s.kind == PP_SRC_EXTENDED_CHERENKOV ||
+                             s.kind == PP_SRC_EXTENDED_CHERENKOV_NO_SMEAR ||
+                             s.kind == PP_SRC_EXTENDED_CHERENKOV_CUSTOM_SMEAR;
+        if (cascade && !(s.long_a > 0.0f && s.long_b > 0.0f && s.long_lrad > 0.0f))
+            return fail(PP_ERR_INVALID, "pp_propagate: cascade needs long_a, long_b, lrad > 0");
+        if (s.kind == PP_SRC_EXTENDED_CHERENKOV_CUSTOM_SMEAR &&
+            !(s.beta_a > 0.0f && s.beta_b > 0.0f))
+            return fail(PP_ERR_INVALID, "pp_propagate: custom smear needs beta_a, beta_b > 0");
+    }
+    if (int rc = set_dev(ctx)) return rc;
+
+    const int block = ctx->cfg.block_size;
+    const size_t lds = (size_t)ctx->n_targets * sizeof(float4) + (size_t)ctx->spec_n * sizeof(float);
+    auto kern = ctx->medium.phase.kind == PP_PHASE_HG ? propagate_brute<PP_PHASE_HG>
+                                                     : propagate_brute<PP_PHASE_MIXED_HG_ES>;
+    int occ = 0;
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds));
+    if (occ < 1) return fail(PP_ERR_INVALID, "pp_propagate: kernel does not fit one block per CU");
+    int64_t max_blocks = (int64_t)occ * ctx->n_cu;
+    if (ctx->cfg.waves_per_simd > 0)
+        max_blocks = std::max<int64_t>(1, (int64_t)ctx->n_cu * 4 * ctx->cfg.waves_per_simd * 64 / block);
+
+    PP_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(Counters), ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    ctx->n_launch = 0;
+    KParams kp{};
+    kp.med = ctx->medium;
+    kp.k0 = (uint32_t)seed;
+    kp.k1 = (uint32_t)(seed >> 32);
+    kp.n_steps = n_steps;
+    kp.targets = ctx->d_targets;
+    kp.n_targets = ctx->n_targets;
+    kp.spec_n = ctx->spec_n;
+    kp.spec_map = ctx->spec_map;
+    kp.spec_mono = ctx->spec_mono;
+    kp.spec = ctx->d_spec;
+    kp.hits = ctx->hits;
+    kp.counters = ctx->d_counters;
+    for (int i = 0; i < n_sources; ++i) {
+        // photon count of src/photon_prop_cuda.jl:456 (Float32 product)
+        const float prod = (float)sources[i].photons * photon_scaling;
+        const int64_t nph = (int64_t)std::floor(prod);
+        if (nph <= 0) continue;
+        kp.src = sources[i];
+        kp.src_ix = (uint32_t)i;
+        kp.n_photons = (uint64_t)nph;
+        const int64_t need = (nph + block - 1) / block;
+        const int grid = (int)std::min<int64_t>(need, max_blocks);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, ctx->stream, kp);
+        PP_HIP(hipGetLastError());
+        ++ctx->n_launch;
+    }
+    PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    Counters hc{};
+    PP_HIP(hipMemcpyAsync(&hc, ctx->d_counters, sizeof hc, hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    PP_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    ctx->n_hits = (int64_t)std::min<unsigned long long>(hc.n_hits, ctx->hits.capacity);
+    ctx->n_ph_sim = (int64_t)hc.n_ph_sim;
+    ctx->perm_valid = false;
+    if (n_hits_out) *n_hits_out = ctx->n_hits;
+    if (n_ph_sim_out) *n_ph_sim_out = ctx->n_ph_sim;
+    if (hc.err) return fail(PP_ERR_HIT_OVERFLOW, "hit buffer overflow (PhotonPropOOMException)");
+    return PP_OK;
+}
+
+int pp_last_timing(pp_ctx* ctx, float* kernel_ms, int32_t* n_launches) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_last_timing: NULL ctx");
+    if (kernel_ms) *kernel_ms = ctx->last_ms;
+    if (n_launches) *n_launches = ctx->n_launch;
+    return PP_OK;
+}
+
+void* pp_stream(pp_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3,
+                 float* init_dir3, float* wavelength, void* time, int time_f64,
+                 float* dist_travelled, int64_t* module_ix, int32_t* n_steps,
+                 uint64_t* photon_id, int32_t* source_ix) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_copy_hits: NULL ctx");
+    if (first < 0 || n < 0 || first + n > ctx->n_hits)
+        return fail(PP_ERR_INVALID, "pp_copy_hits: range outside [0, n_hits)");
+    if (n == 0) return PP_OK;
+    if (int rc = set_dev(ctx)) return rc;
+    const bool ordered = ctx->cfg.deterministic_order != 0;
+    // rows of the device buffer to read: [lo, lo+cnt)
+    int64_t lo = first, cnt = n;
+    if (ordered) {
+        if (!ctx->perm_valid) {
+            std::vector<uint64_t> pid((size_t)ctx->n_hits);
+            std::vector<int32_t> src((size_t)ctx->n_hits);
+            PP_HIP(hipMemcpy(pid.data(), ctx->hits.photon_id, pid.size() * 8, hipMemcpyDeviceToHost));
+            PP_HIP(hipMemcpy(src.data(), ctx->hits.source_ix, src.size() * 4, hipMemcpyDeviceToHost));
+            ctx->perm.resize((size_t)ctx->n_hits);
+            std::iota(ctx->perm.begin(), ctx->perm.end(), (int64_t)0);
+            std::sort(ctx->perm.begin(), ctx->perm.end(), [&](int64_t a, int64_t b) {
+                return src[a] != src[b] ? src[a] < src[b] : pid[a] < pid[b];
+            });
+            ctx->perm_valid = true;
+        }
+        lo = 0;
+        cnt = ctx->n_hits;
+    }
+    auto row = [&](int64_t k) -> int64_t {
+        return ordered ? ctx->perm[(size_t)(first + k)] : (first + k - lo);
+    };
+    std::vector<float> fbuf((size_t)cnt);
+    auto get_f = [&](int col) -> int {
+        PP_HIP(hipMemcpy(fbuf.data(), ctx->hits.f[col] + lo, (size_t)cnt * 4, hipMemcpyDeviceToHost));
+        return PP_OK;
+    };
+    float* vec_out[3] = {pos3, dir3, init_dir3};
+    for (int v = 0; v < 3; ++v) {
+        if (!vec_out[v]) continue;
+        for (int c = 0; c < 3; ++c) {
+            if (int rc = get_f(3 * v + c)) return rc;
+            for (int64_t k = 0; k < n; ++k) vec_out[v][3 * k + c] = fbuf[(size_t)row(k)];
+        }
+    }
+    if (wavelength) {
+        if (int rc = get_f(9)) return rc;
+        for (int64_t k = 0; k < n; ++k) wavelength[k] = fbuf[(size_t)row(k)];
+    }
+    if (time) {
+        if (int rc = get_f(10)) return rc;
+        for (int64_t k = 0; k < n; ++k) {
+            if (time_f64)
+                ((double*)time)[k] = fbuf[(size_t)row(k)];
+            else
+                ((float*)time)[k] = fbuf[(size_t)row(k)];
+        }
+    }
+    if (dist_travelled) {
+        if (int rc = get_f(11)) return rc;
+        for (int64_t k = 0; k < n; ++k) dist_travelled[k] = fbuf[(size_t)row(k)];
+    }
+    if (module_ix || n_steps || source_ix) {
+        std::vector<int32_t> ibuf((size_t)cnt);
+        int32_t* cols[3] = {ctx->hits.module_ix, ctx->hits.n_steps, ctx->hits.source_ix};
+        for (int c = 0; c < 3; ++c) {
+            if ((c == 0 && !module_ix) || (c == 1 && !n_steps) || (c == 2 && !source_ix)) continue;
+            PP_HIP(hipMemcpy(ibuf.data(), cols[c] + lo, (size_t)cnt * 4, hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < n; ++k) {
+                const int32_t v = ibuf[(size_t)row(k)];
+                if (c == 0) module_ix[k] = v;
+                else if (c == 1) n_steps[k] = v;
+                else source_ix[k] = v;
+            }
+        }
+    }
+    if (photon_id) {
+        std::vector<uint64_t> ubuf((size_t)cnt);
+        PP_HIP(hipMemcpy(ubuf.data(), ctx->hits.photon_id + lo, (size_t)cnt * 8, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < n; ++k) photon_id[k] = ubuf[(size_t)row(k)];
+    }
+    return PP_OK;
+}
+
+int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, int out_stride,
+               int64_t n) {
+    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1};
+    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1};
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_eval_fn: NULL ctx");
+    if (fn < 0 || fn >= PP_FN_COUNT) return fail(PP_ERR_INVALID, "pp_eval_fn: unknown function");
+    if (n < 0 || (n > 0 && (!in || !out)))
+        return fail(PP_ERR_INVALID, "pp_eval_fn: bad buffers");
+    if (in_stride < kIn[fn] || out_stride < kOut[fn])
+        return fail(PP_ERR_INVALID, "pp_eval_fn: stride smaller than the function's arity");
+    if (fn == PP_FN_CASCADIA && !ctx->has_medium)
+        return fail(PP_ERR_STATE, "pp_eval_fn: set a medium first");
+    if (fn == PP_FN_SPECTRUM && !ctx->has_spectrum)
+        return fail(PP_ERR_STATE, "pp_eval_fn: set a spectrum first");
+    if (n == 0) return PP_OK;
+    if (int rc = set_dev(ctx)) return rc;
+    float *d_in = nullptr, *d_out = nullptr;
+    const size_t in_b = (size_t)n * in_stride * 4, out_b = (size_t)n * out_stride * 4;
+    PP_HIP(hipMalloc(&d_in, in_b));
+    if (hipMalloc(&d_out, out_b) != hipSuccess) {
+        (void)hipFree(d_in);
+        return fail(PP_ERR_NOMEM, "pp_eval_fn: output allocation");
+    }
+    int rc = PP_OK;
+    do {
+        if (hipMemcpy(d_in, in, in_b, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(PP_ERR_HIP, "copy in"); break; }
+        if (hipMemset(d_out, 0, out_b) != hipSuccess) { rc = fail(PP_ERR_HIP, "memset"); break; }
+        EvalParams e{};
+        e.fn = fn;
+        e.in_stride = in_stride;
+        e.out_stride = out_stride;
+        e.n = n;
+        e.in = d_in;
+        e.out = d_out;
+        e.med = ctx->medium;
+        e.spec = ctx->d_spec;
+        e.spec_n = ctx->spec_n;
+        e.spec_map = ctx->spec_map;
+        e.spec_mono = ctx->spec_mono;
+        const int64_t grid = (n + 255) / 256;
+        hipLaunchKernelGGL(eval_fn_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, e);
+        if (hipGetLastError() != hipSuccess) { rc = fail(PP_ERR_HIP, "eval launch"); break; }
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) { rc = fail(PP_ERR_HIP, "eval sync"); break; }
+        if (hipMemcpy(out, d_out, out_b, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(PP_ERR_HIP, "copy out"); break; }
+    } while (0);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+}  // extern "C"

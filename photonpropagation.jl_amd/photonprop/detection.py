@@ -1,0 +1,76 @@
+"""Targets: spherical optical modules.
+
+Mirror of the NeutrinoTelescopeBase target types the reference passes to the
+kernel: ``Spherical(position, radius)`` shapes wrapped in targets that carry a
+``module_id`` (src/processing.jl:57-70 reads ``targ.shape`` and
+``targ.module_id``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class Spherical:
+    position: np.ndarray
+    radius: float
+
+    def surface_area(self):
+        return 4.0 * np.pi * float(self.radius) ** 2
+
+
+@dataclass
+class PhotonTarget:
+    shape: Spherical
+    module_id: int
+
+
+@dataclass
+class HomogeneousDetector(PhotonTarget):
+    """HomogeneousDetector(shape, active_area, module_id) (README.md:34-38)."""
+
+    def __init__(self, shape, active_area, module_id):
+        self.shape = shape
+        self.active_area = float(active_area)
+        self.module_id = int(module_id)
+
+
+def spherical_target(position, radius, module_id):
+    return PhotonTarget(Spherical(np.asarray(position, dtype=np.float32), float(radius)),
+                        int(module_id))
+
+
+def targets_to_pod(targets):
+    """Lower a target list to the pp_target array (index order preserved)."""
+    arr = (_lib.pp_target * max(len(targets), 1))()
+    for i, t in enumerate(targets):
+        shape = t.shape if hasattr(t, "shape") else t
+        if not isinstance(shape, Spherical):
+            raise TypeError(f"target {i}: only Spherical shapes are supported")
+        pos = np.asarray(shape.position, dtype=np.float32).reshape(3)
+        arr[i].position[:] = [float(x) for x in pos]
+        arr[i].radius = float(np.float32(shape.radius))
+        arr[i].shape_kind = _lib.SHAPE_SPHERE
+        arr[i].module_id = int(getattr(t, "module_id", i + 1))
+    return arr
+
+
+def make_detector_grid(n_x, n_y, spacing, n_modules, v_spacing, radius=0.3, z_center=0.0):
+    """Strings on an n_x x n_y grid (centred at the origin), each with
+    n_modules spheres at v_spacing, centred on z_center. Module ids 1..N in
+    string-major order."""
+    xs = (np.arange(n_x) - (n_x - 1) / 2.0) * spacing
+    ys = (np.arange(n_y) - (n_y - 1) / 2.0) * spacing
+    zs = (np.arange(n_modules) - (n_modules - 1) / 2.0) * v_spacing + z_center
+    targets = []
+    mid = 1
+    for x in xs:
+        for y in ys:
+            for z in zs:
+                targets.append(spherical_target((x, y, z), radius, mid))
+                mid += 1
+    return targets

@@ -1,0 +1,177 @@
+"""GPU step functions (pp_eval_fn, the kernel's own inlined code) against the
+CPU oracle on identical inputs.
+
+Tolerance: 4 FP32 ulp of the output's scale. Unit-vector components are
+measured at scale 1 (ulp(1) = 2^-23); scalars at their own magnitude.
+Philox and the spectrum lookup are bit-exact."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+ULP1 = 2.0**-23
+
+
+def ulp(x):
+    x = np.maximum(np.abs(np.asarray(x, dtype=np.float64)), 2.0**-126)
+    return 2.0 ** (np.floor(np.log2(x)) - 23)
+
+
+def report(name, err):
+    print(f"{name}: max {np.max(err):.2f} ulp, p99.9 {np.quantile(err, 0.999):.2f} ulp")
+
+
+def unit(n):
+    v = RNG.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def both(P, O, dev, fn, x, **kw):
+    g = P._lib.eval_fn(dev, fn, x).astype(np.float64)
+    r = O.eval_fn(fn, x, **kw).astype(np.float64)
+    return g, r
+
+
+def test_philox_bit_exact(P, O, dev):
+    w = RNG.integers(0, 2**32, (100_000, 6), dtype=np.uint64).astype(np.uint32)
+    g = P._lib.eval_fn(dev, "PHILOX", w.view(np.float32)).view(np.uint32)
+    r = O.eval_fn("PHILOX", w.view(np.float32)).view(np.uint32)
+    np.testing.assert_array_equal(g, r)
+
+
+def test_isotropic_dir(P, O, dev):
+    x = RNG.random((200_000, 2), dtype=np.float32)
+    g, r = both(P, O, dev, "ISOTROPIC_DIR", x)
+    err = np.abs(g - r) / ULP1
+    report("isotropic_dir", err)
+    assert err.max() <= 4
+
+
+def test_rot_from_ez(P, O, dev):
+    n = 200_000
+    x = np.concatenate([unit(n), unit(n)], axis=1)
+    x[:10, :3] = [0, 0, 1]
+    x[10:20, :3] = [0, 0, -1]
+    g, r = both(P, O, dev, "ROT_FROM_EZ", x)
+    err = np.abs(g - r) / ULP1
+    report("rot_from_ez", err)
+    assert err.max() <= 4
+
+
+@pytest.mark.parametrize("g_", [0.5, 0.95, 0.99])
+def test_hg_cos(P, O, dev, g_):
+    u = RNG.random(200_000, dtype=np.float32)
+    x = np.stack([np.full_like(u, g_), u], 1)
+    g, r = both(P, O, dev, "HG_COS", x)
+    err = np.abs(g - r) / ULP1  # cosines: scale 1
+    report(f"hg_cos g={g_}", err)
+    assert err.max() <= 4
+
+
+@pytest.mark.parametrize("b", [0.835, 0.3])
+def test_es_cos(P, O, dev, b):
+    u = RNG.random(200_000, dtype=np.float32)
+    x = np.stack([np.full_like(u, b), u], 1)
+    g, r = both(P, O, dev, "ES_COS", x)
+    err = np.abs(g - r) / ULP1
+    report(f"es_cos b={b}", err)
+    assert err.max() <= 4
+
+
+def test_update_direction(P, O, dev):
+    n = 200_000
+    x = np.concatenate([unit(n), RNG.uniform(-1, 1, (n, 1)).astype(np.float32),
+                        RNG.random((n, 1), dtype=np.float32)], 1)
+    g, r = both(P, O, dev, "UPDATE_DIR", x)
+    err = np.abs(g - r) / ULP1
+    report("update_direction", err)
+    assert err.max() <= 4
+
+
+def test_sphere_intersection(P, O, dev):
+    n = 200_000
+    c = RNG.uniform(-20, 20, (n, 3)).astype(np.float32)
+    r = RNG.uniform(0.1, 1.0, n).astype(np.float32)
+    p = RNG.uniform(-30, 30, (n, 3)).astype(np.float32)
+    aim = c + RNG.normal(0, 0.5, (n, 3)).astype(np.float32) - p
+    d = (aim / np.linalg.norm(aim, axis=1, keepdims=True)).astype(np.float32)
+    step = RNG.exponential(40.0, n).astype(np.float32)
+    x = np.concatenate([c, r[:, None], p, d, step[:, None]], 1)
+    g, ref = both(P, O, dev, "SPHERE_ISEC", x)
+    # exact geometry in float64 to locate ill-conditioned rows (grazing rays,
+    # roots at the step boundary)
+    pd, cd, dd = p.astype(np.float64), c.astype(np.float64), d.astype(np.float64)
+    a = np.sum(dd * (pd - cd), 1)
+    disc = a * a - (np.sum((pd - cd) ** 2, 1) - r.astype(np.float64) ** 2)
+    root = -a - np.sqrt(np.abs(disc))
+    edge = (np.abs(disc) < 1e-3 * r**2) | (np.abs(root - step) < 1e-4 * step) | (
+        np.abs(root) < 1e-4)
+    same = g[:, 0] == ref[:, 0]
+    assert np.all(same | edge)
+    h = (g[:, 0] == 1) & (ref[:, 0] == 1) & ~edge
+    assert h.sum() > 10_000
+    # d = -a - sqrt(b): 4 ulp of the larger operand |a|
+    err = np.abs(g[h, 1] - ref[h, 1]) / ulp(np.abs(a[h]))
+    report("sphere d", err)
+    assert err.max() <= 4
+
+
+def test_step_length(P, O, dev):
+    u = ((RNG.integers(0, 2**24, 200_000) + 1) * 2.0**-24).astype(np.float32)
+    x = np.stack([u, RNG.uniform(5, 100, len(u)).astype(np.float32)], 1)
+    g, r = both(P, O, dev, "STEP_LEN", x)
+    nz = r[:, 0] > 0
+    err = np.abs(g[nz] - r[nz]) / ulp(r[nz])
+    report("step_length", err)
+    assert err.max() <= 4
+    assert np.all(g[~nz] == 0)
+
+
+def test_cascadia_medium_values(P, O, dev):
+    m = P.CascadiaMediumProperties()
+    P.processing.stage(dev, [P.spherical_target((0, 0, 0), 1.0, 1)], m, P.Monochromatic(450.0))
+    wl = np.linspace(290.0, 810.0, 20_001, dtype=np.float32)
+    g, r = both(P, O, dev, "CASCADIA", wl[:, None], medium=m.to_pod())
+    for k, name in enumerate(("sca_len", "c_grp", "n_phase")):
+        err = np.abs(g[:, k] - r[:, k]) / ulp(r[:, k])
+        report(name, err)
+        assert err.max() <= 4, name
+
+
+def test_cherenkov_track_direction(P, O, dev):
+    x = RNG.random((200_000, 2), dtype=np.float32)
+    g, r = both(P, O, dev, "TRACK_DIR", x)
+    err = np.abs(g - r) / ULP1
+    report("track_dir", err)
+    assert err.max() <= 4
+
+
+def test_gamma_sampler_same_draws(P, O, dev):
+    n = 50_000
+    rows = np.zeros((n, 7), dtype=np.float32)
+    rows[:, 0] = 7.837
+    rows[:, 1] = 1 / 0.632
+    w = rows.view(np.uint32)
+    w[:, 2] = 99
+    w[:, 4] = np.arange(n)
+    g, r = both(P, O, dev, "GAMMA", rows)
+    err = np.abs(g[:, 0] - r[:, 0]) / ulp(r[:, 0])
+    report("gamma", err)
+    # identical accept/reject decisions give values within a few ulp; a flipped
+    # decision (boundary case) gives a different draw
+    assert np.mean(err <= 8) > 0.9999
+
+
+def test_spectrum_lookup_bit_exact(P, O, dev):
+    m = P.CascadiaMediumProperties()
+    tab = P.workloads.acceptance_weighted_spectrum(m)
+    for mapping in (0, 1):
+        spec = P.DeviceSpectralDist(tab.table, tab.normalization, mapping)
+        P.processing.stage(dev, [P.spherical_target((0, 0, 0), 1.0, 1)], m, spec)
+        u = RNG.random((100_000, 1), dtype=np.float32)
+        g = P._lib.eval_fn(dev, "SPECTRUM", u)
+        r = O.eval_fn("SPECTRUM", u, spectrum=(0.0, tab.table, mapping))
+        err = np.abs(g - r) / ulp(r)
+        report(f"spectrum mapping={mapping}", err)
+        assert err.max() <= 1

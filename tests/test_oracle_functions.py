@@ -1,0 +1,274 @@
+"""Pins every oracle step function against mathematics fixed independently of
+the oracle's own formula: closed-form CDFs evaluated at the sampled values,
+moments, invariants, scipy distributions and the reference-held values of
+examples/axicone.ipynb (cell 2)."""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from conftest import GOLDEN
+
+RNG = np.random.default_rng(12345)
+
+
+def grid_u(n):
+    return ((np.arange(n) + 0.5) / n).astype(np.float32)
+
+
+# ---- directions -------------------------------------------------------------------------
+
+def test_isotropic_direction_distribution(O):
+    u = RNG.random((200_000, 2), dtype=np.float32)
+    d = O.eval_fn("ISOTROPIC_DIR", u).astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-6)
+    assert stats.kstest(d[:, 2], stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    phi = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi)
+    assert stats.kstest(phi, stats.uniform(0, 2 * np.pi).cdf).pvalue > 1e-3
+    # draw order: u1 -> cos(theta) = 2 u1 - 1, u2 -> phi = 2 pi u2
+    np.testing.assert_allclose(d[:, 2], 2 * u[:, 0].astype(np.float64) - 1, atol=2e-6)
+    np.testing.assert_allclose(np.cos(phi), np.cos(2 * np.pi * u[:, 1]), atol=5e-5)
+    e = O.eval_fn("ISOTROPIC_DIR", [[0.0, 0.0], [0.5, 0.25], [1.0 - 2**-24, 0.0]])
+    np.testing.assert_allclose(e, [[0, 0, -1], [0, 1, 0], [0, 0, 1]], atol=1e-3)
+
+
+def random_unit(n):
+    v = RNG.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def rot(O, a, b):
+    return O.eval_fn("ROT_FROM_EZ", np.concatenate([a, b], axis=1)).astype(np.float64)
+
+
+def test_rot_from_ez_invariants(O):
+    n = 50_000
+    a = random_unit(n)
+    a[:100, 2] = -np.abs(a[:100, 2])  # both hemispheres well covered
+    b = random_unit(n)
+    ez = np.tile(np.float32([0, 0, 1]), (n, 1))
+    np.testing.assert_allclose(rot(O, a, ez), a, atol=3e-6)  # R e_z = a
+    rb = rot(O, a, b)
+    np.testing.assert_allclose(np.linalg.norm(rb, axis=1), 1.0, atol=3e-6)
+    np.testing.assert_allclose(np.sum(rb * a, axis=1), b[:, 2], atol=3e-6)  # angle to axis kept
+    ex = np.tile(np.float32([1, 0, 0]), (n, 1))
+    ey = np.tile(np.float32([0, 1, 0]), (n, 1))
+    rx, ry = rot(O, a, ex), rot(O, a, ey)
+    np.testing.assert_allclose(np.cross(rx, ry), a, atol=5e-6)  # proper rotation
+    np.testing.assert_allclose(np.sum(rx * ry, axis=1), 0.0, atol=3e-6)
+    # minimal rotation: the axis e_z x a is left fixed
+    k = np.cross(ez.astype(np.float64), a)
+    kn = np.linalg.norm(k, axis=1)
+    ok = kn > 1e-3
+    k = (k[ok] / kn[ok, None]).astype(np.float32)
+    np.testing.assert_allclose(rot(O, a[ok], k), k, atol=5e-6)
+
+
+def test_rot_from_ez_poles(O):
+    b = np.float32([[0.3, -0.4, 0.866025]])
+    np.testing.assert_allclose(rot(O, np.float32([[0, 0, 1]]), b), b, atol=1e-7)
+    np.testing.assert_allclose(rot(O, np.float32([[0, 0, -1]]), b),
+                               [[0.3, 0.4, -0.866025]], atol=1e-7)
+
+
+# ---- phase functions -----------------------------------------------------------------
+
+def hg_cdf(x, g):
+    return (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * x) - 1 / (1 + g))
+
+
+def es_cdf(x, b):
+    return (x + b * x**3 / 3 + 1 + b / 3) / (2 + 2 * b / 3)
+
+
+@pytest.mark.parametrize("g", [0.5, 0.9, 0.95, 0.99])
+def test_hg_inverse_cdf_and_mean(O, g):
+    u = grid_u(200_000)
+    inp = np.stack([np.full_like(u, g), u], axis=1)
+    x = O.eval_fn("HG_COS", inp)[:, 0].astype(np.float64)
+    assert np.all(np.abs(x) <= 1)
+    # the sample at u inverts the closed-form HG CDF, to float32 resolution of
+    # x scaled by the CDF's slope (the pdf)
+    pdf = (1 - g * g) / (2 * (1 + g * g - 2 * g * x) ** 1.5)
+    tol = 4 * 2.0**-24 * pdf + 1e-6
+    assert np.all(np.abs(hg_cdf(x, g) - u) <= tol)
+    assert abs(x.mean() - g) < 2e-3
+
+
+@pytest.mark.parametrize("b", [0.3, 0.835, 2.0])
+def test_es_inverse_cdf_and_moments(O, b):
+    u = grid_u(200_000)
+    x = O.eval_fn("ES_COS", np.stack([np.full_like(u, b), u], axis=1))[:, 0].astype(np.float64)
+    np.testing.assert_allclose(es_cdf(x, b), u, atol=2e-7)
+    assert abs(x.mean()) < 1e-4
+    assert abs((x**2).mean() - (1 / 3 + b / 5) / (1 + b / 3)) < 1e-4
+
+
+def test_mixed_phase_function_mean(O, P):
+    m = P.CascadiaMediumProperties().to_pod()
+    u = RNG.random((400_000, 2), dtype=np.float32)
+    # branch choice u0 < hg_fraction = 0.8 -> HG(0.95), else ES(0.835)
+    hg = O.eval_fn("HG_COS", np.stack([np.full(len(u), 0.95, np.float32), u[:, 1]], 1))[:, 0]
+    es = O.eval_fn("ES_COS", np.stack([np.full(len(u), 0.835, np.float32), u[:, 1]], 1))[:, 0]
+    x = np.where(u[:, 0] < m.hg_fraction, hg, es)
+    assert abs(x.mean() - 0.76) < 5e-3
+
+
+def test_update_direction_angle_and_azimuth(O):
+    n = 50_000
+    d = random_unit(n)
+    ct = RNG.uniform(-1, 1, n).astype(np.float32)
+    up = RNG.random(n, dtype=np.float32)
+    out = O.eval_fn("UPDATE_DIR", np.concatenate([d, ct[:, None], up[:, None]], 1))
+    out = out.astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(out, axis=1), 1, atol=3e-6)
+    np.testing.assert_allclose(np.sum(out * d, axis=1), ct, atol=3e-6)
+
+
+# ---- intersection ----------------------------------------------------------------------
+
+def near_root(c, r, p, d):
+    dp = p - c
+    a = np.sum(d * dp, axis=1)
+    disc = a * a - (np.sum(dp * dp, axis=1) - r * r)
+    root = -a - np.sqrt(np.abs(disc))
+    return np.where(disc >= 0, root, -1.0)
+
+
+def test_sphere_intersection_closed_form(O):
+    n = 100_000
+    c = RNG.uniform(-5, 5, (n, 3)).astype(np.float32)
+    r = RNG.uniform(0.1, 2.0, n).astype(np.float32)
+    p = RNG.uniform(-10, 10, (n, 3)).astype(np.float32)
+    d = random_unit(n)
+    # half of the rays aim near the sphere centre
+    aim = c[: n // 2] + RNG.normal(0, 0.7, (n // 2, 3)).astype(np.float32) - p[: n // 2]
+    d[: n // 2] = (aim / np.linalg.norm(aim, axis=1, keepdims=True)).astype(np.float32)
+    step = RNG.exponential(10.0, n).astype(np.float32)
+    out = O.eval_fn("SPHERE_ISEC", np.concatenate([c, r[:, None], p, d, step[:, None]], 1))
+    ref = near_root(c.astype(np.float64), r.astype(np.float64), p.astype(np.float64),
+                    d.astype(np.float64))
+    hit_ref = (ref > 0) & (ref < step)
+    # decisions agree except within float rounding of the boundaries
+    margin = (np.abs(ref) < 1e-3) | (np.abs(ref - step) < 1e-3)
+    assert np.all(((out[:, 0] == 1) == hit_ref) | margin)
+    h = (out[:, 0] == 1) & ~margin
+    assert h.sum() > 1000
+    np.testing.assert_allclose(out[h, 1], ref[h], rtol=1e-4, atol=1e-4)
+    # the hit point lies on the sphere
+    hp = p[h] + out[h, 1:2] * d[h]
+    np.testing.assert_allclose(np.linalg.norm(hp - c[h], axis=1), r[h], rtol=1e-3, atol=1e-4)
+    # photons starting inside a sphere never hit it
+    inside = np.linalg.norm(p - c, axis=1) < r
+    assert not np.any(out[inside, 0] == 1)
+
+
+def test_sphere_intersection_cases(O):
+    rows = [
+        # c, r, pos, dir, step -> hit, d
+        ([0, 0, 10], 0.3, [0, 0, 0], [0, 0, 1], 100.0, 1, 9.7),
+        ([0, 0, 10], 0.3, [0, 0, 0], [0, 0, 1], 9.0, 0, 0.0),     # too short
+        ([0, 0, 10], 0.3, [0, 0, 0], [0, 0, -1], 100.0, 0, 0.0),  # pointing away
+        ([0, 0, 10], 0.3, [0, 0, 10], [0, 0, 1], 100.0, 0, 0.0),  # inside
+        ([0, 0, 10], 0.3, [0.5, 0, 0], [0, 0, 1], 100.0, 0, 0.0),  # misses
+    ]
+    inp = [[*c, r, *p, *d, s] for c, r, p, d, s, _, _ in rows]
+    out = O.eval_fn("SPHERE_ISEC", inp)
+    for (c, r, p, d, s, h, dist), o in zip(rows, out):
+        assert o[0] == h
+        assert abs(o[1] - dist) < 1e-5
+
+
+# ---- step length, media ----------------------------------------------------------------
+
+def test_step_length_exponential(O):
+    u = (RNG.integers(0, 2**24, 100_000) + 1).astype(np.float32) * np.float32(2**-24)
+    s = O.eval_fn("STEP_LEN", np.stack([u, np.full_like(u, 48.4)], 1))[:, 0]
+    assert np.all(s >= 0)
+    assert stats.kstest(s, stats.expon(scale=48.4).cdf).pvalue > 1e-3
+
+
+def test_cascadia_reference_values(O, P):
+    """examples/axicone.ipynb cell 2 prints the Cascadia lengths at 488 nm."""
+    ref = json.load(open(os.path.join(GOLDEN, "reference_values.json")))["cascadia_488nm"]
+    m = P.CascadiaMediumProperties().to_pod()
+    out = O.eval_fn("CASCADIA", [[488.0]], medium=m)[0]
+    assert abs(out[0] - ref["scattering_length_m"]) < 1e-5 * ref["scattering_length_m"]
+    # host-side absorption length (STRAW table) used for the hit weights
+    med = P.CascadiaMediumProperties()
+    assert abs(med.absorption_length(488.0) - ref["absorption_length_m"]) < 1e-5
+    assert abs(med.scattering_length(488.0) - ref["scattering_length_m"]) < 1e-4
+
+
+def test_quan_fry_invariants(O, P):
+    """Parity unpinned (no reference value); physical invariants only."""
+    wl = np.linspace(300, 800, 101, dtype=np.float32)
+    m = P.CascadiaMediumProperties().to_pod()
+    out = O.eval_fn("CASCADIA", wl[:, None], medium=m).astype(np.float64)
+    sca, cg, n = out[:, 0], out[:, 1], out[:, 2]
+    assert np.all(np.diff(n) < 0)  # normal dispersion
+    assert np.all((n > 1.33) & (n < 1.38))
+    assert np.all(cg < 0.299792458 / n)  # group index above phase index
+    assert np.all(np.diff(sca) > 0)  # scattering weaker at long wavelength
+
+
+def test_simple_medium_passthrough(O, P):
+    med = P.SimpleMediumProperties(30.0, 25.0, 1.35, 0.9, 0.22).to_pod()
+    out = O.eval_fn("CASCADIA", [[350.0], [600.0]], medium=med)
+    np.testing.assert_allclose(out, [[25.0, 0.22, 1.35]] * 2, rtol=1e-7)
+
+
+# ---- emission samplers -----------------------------------------------------------------
+
+def test_cherenkov_track_direction(O):
+    u = RNG.random((200_000, 2), dtype=np.float32)
+    d = O.eval_fn("TRACK_DIR", u).astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1, atol=3e-6)
+    a, b = 0.39, 2.61
+    norm = 1 - np.exp(-b * 2**a)
+    x = 1 - d[:, 2]
+    cdf = lambda t: (1 - np.exp(-b * np.clip(t, 0, 2) ** a)) / norm  # noqa: E731
+    assert stats.kstest(x, cdf).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("shape,scale", [(7.837, 1 / 0.632), (1.776, 1.0), (0.164, 1.0)])
+def test_gamma_sampler(O, shape, scale):
+    n = 20_000
+    rows = np.zeros((n, 7), dtype=np.float32)
+    rows[:, 0] = shape
+    rows[:, 1] = scale
+    w = rows.view(np.uint32)
+    w[:, 2] = 1
+    w[:, 4] = np.arange(n)
+    x = O.eval_fn("GAMMA", rows)[:, 0]
+    assert np.all(x > 0) or shape < 1
+    assert stats.kstest(x, stats.gamma(shape, scale=scale).cdf).pvalue > 1e-3
+
+
+def test_beta_sampler(O):
+    L = O.lib()
+    x = np.array([L.ppo_beta_sample(1.776, 0.164, 7, i, 0) for i in range(20_000)])
+    dist = stats.beta(1.776, 0.164)
+    # b < 1 puts ~7% of the mass within float32 resolution of 1: test the bulk
+    # against the conditional CDF and the tail mass binomially
+    t = 1.0 - 2.0**-18
+    ft = dist.cdf(t)
+    bulk = x[x < t]
+    assert stats.kstest(bulk, lambda v: dist.cdf(v) / ft).pvalue > 1e-3
+    tail = np.sum(x >= t)
+    assert stats.binomtest(int(tail), len(x), 1 - ft).pvalue > 1e-3
+
+
+def test_spectrum_lookup_mappings(O):
+    tab = np.linspace(300, 700, 600).astype(np.float32) + (np.arange(600) % 3).astype(np.float32)
+    k = np.arange(600)
+    u = (k / 599.0).astype(np.float32)
+    got = O.eval_fn("SPECTRUM", u[:, None], spectrum=(0.0, tab, 0))[:, 0]
+    np.testing.assert_allclose(got, tab, rtol=2e-6)
+    ut = ((k + 0.5) / 600.0).astype(np.float32)
+    gt = O.eval_fn("SPECTRUM", ut[:, None], spectrum=(0.0, tab, 1))[:, 0]
+    np.testing.assert_allclose(gt, tab, rtol=2e-6)
+    mono = O.eval_fn("SPECTRUM", [[0.3]], spectrum=(450.0, None, 0))
+    assert mono[0, 0] == 450.0
