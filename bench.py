@@ -32,6 +32,13 @@ W_PHYS_TRANSC = 7.4
 #   one _intersect_nsphere test (src/photon_prop_cuda.jl:309-343)
 W_ISEC_FLOP = 20.0
 W_ISEC_TRANSC = 1.0
+# Grid search (this build): per step the segment is clipped to the grid box
+# (3 reciprocals, 6 slab products, 12 min/max, entry cell and boundary
+# parameters: 25 FP32 ops); per visited cell the walk compares and advances
+# (min3, 2 compares, 1 add: 4 FP32 ops).
+W_GRID_STEP_FLOP = 25.0
+W_GRID_STEP_TRANSC = 3.0
+W_CELL_FLOP = 4.0
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # without packed-FP32 instructions
 
@@ -43,6 +50,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--search", choices=["auto", "brute", "grid"], default="auto")
+    ap.add_argument("--cell", type=float, default=0.0, help="grid cell edge in m (0 = auto)")
+    ap.add_argument("--block", type=int, default=0, help="threads per block (0 = library default)")
+    ap.add_argument("--wps", type=int, default=0, help="waves per SIMD for grid sizing (0 = auto)")
     ap.add_argument("--cpu-sample", type=float, default=0.0,
                     help="photons for the CPU baseline (0 = calibrate to ~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -99,6 +109,9 @@ def main():
     wl = P.workloads.cascade_16_strings(photons=nph)
     cfg = P._lib.pp_config()
     cfg.search = {"auto": 0, "brute": 1, "grid": 2}[args.search]
+    cfg.grid_cell = args.cell
+    cfg.block_size = args.block
+    cfg.waves_per_simd = args.wps
     cfg.deterministic_order = 0
     _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
     P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
@@ -125,13 +138,35 @@ def main():
     hbc = P.HostHitBuffer(np.float32)
     hits, nsim = P.run_photon_prop_no_local_cache(wl.sources, wl.targets, wl.medium, wl.spectrum,
                                                   wl.seed, hbc, dev, n_steps=wl.n_steps)
-    steps = total_steps(hits, nsim, wl.n_steps)
     kms = float(np.mean(kernel_ms))
+    search_used = ctypes.c_int32()
+    lib.pp_search_info(dev.handle, ctypes.byref(search_used), None, None, None)
+    grid = search_used.value == 2
+    # work actually executed: one instrumented run (identical hits), untimed
+    ccfg = P._lib.pp_config()
+    ccfg.search = cfg.search
+    ccfg.grid_cell = cfg.grid_cell
+    ccfg.block_size = cfg.block_size
+    ccfg.waves_per_simd = cfg.waves_per_simd
+    ccfg.count_work = 1
+    _, cdev = P.make_hit_buffers(capacity=1 << 24, config=ccfg)
+    P.processing.stage(cdev, wl.targets, wl.medium, wl.spectrum)
+    P._lib.check(lib.pp_propagate(cdev.handle, pods, 1, ctypes.c_uint64(wl.seed), wl.n_steps,
+                                  1.0, ctypes.byref(n_hits), ctypes.byref(n_sim)),
+                 "pp_propagate (count_work)")
+    w_steps, w_tests, w_cells = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    P._lib.check(lib.pp_work_counts(cdev.handle, ctypes.byref(w_steps), ctypes.byref(w_tests),
+                                    ctypes.byref(w_cells)), "pp_work_counts")
+    cdev.close()
+    steps, tests, cells = w_steps.value, w_tests.value, w_cells.value
+    assert steps == total_steps(hits, nsim, wl.n_steps)
     n_t = len(wl.targets)
-    tbar = n_t  # sphere tests per step of the reference's brute-force search
-    flop = steps * (W_PHYS_FLOP + W_ISEC_FLOP * tbar)
+    flop = (steps * (W_PHYS_FLOP + (W_GRID_STEP_FLOP if grid else 0.0)) + tests * W_ISEC_FLOP
+            + cells * W_CELL_FLOP)
+    transc = (steps * (W_PHYS_TRANSC + (W_GRID_STEP_TRANSC if grid else 0.0))
+              + tests * W_ISEC_TRANSC)
     achieved = flop / (kms * 1e-3) / 1e12
-    transc = steps * (W_PHYS_TRANSC + W_ISEC_TRANSC * tbar) / (kms * 1e-3)
+    ref_equiv = steps * (W_PHYS_FLOP + W_ISEC_FLOP * n_t) / (kms * 1e-3) / 1e12
     out = {
         "metric": "photons propagated/s on 1 MI355X",
         "value": photons_per_s,
@@ -146,18 +181,24 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "cascade_16_strings", "photons": nph, "targets": n_t,
-                   "n_steps": wl.n_steps, "search": args.search,
+                   "n_steps": wl.n_steps, "search": args.search, "cell_m": args.cell,
+                   "block": args.block, "waves_per_simd": args.wps,
                    "source": "ExtendedCherenkovEmitter 10 TeV e-, 4x4x20 spheres r=0.3 m"},
         "photon_steps_per_s": steps / (ms_per_step / 1e3),
         "kernel_ms": kms,
         "hits_per_step": int(len(hits["n_steps"])),
+        "sphere_tests_per_step": tests / steps,
+        "cell_visits_per_step": cells / steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": None,
                      "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
-                     "transcendental_per_s": transc,
+                     "transcendental_per_s": transc / (kms * 1e-3),
                      "flop_per_launch": flop,
-                     "basis": "reference-equivalent FP32 ops: (47 + 20*320) per photon step"},
+                     "reference_equivalent_tflops": ref_equiv,
+                     "basis": "FP32 ops executed: 47/step (reference step) + 25/step grid "
+                              "clip + 20/sphere test + 4/cell visit, counted by the "
+                              "instrumented kernel"},
     }
     dev.close()
     if not args.no_cpu:

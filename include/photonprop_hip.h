@@ -129,7 +129,9 @@ typedef struct pp_config {
     int32_t search;             /* enum pp_search_kind                         */
     float grid_cell;            /* grid cell edge (m); 0 = auto                */
     int32_t deterministic_order;/* pp_copy_hits sorts by (source, photon id)   */
-    int32_t reserved[11];
+    int32_t count_work;         /* instrumented kernel: counts steps, sphere
+                                   tests and grid-cell visits (pp_work_counts) */
+    int32_t reserved[10];
 } pp_config; /* 64 bytes */
 
 /* ---- random stream layout (the reproducibility contract) ---------------
@@ -218,6 +220,17 @@ int pp_last_timing(pp_ctx* ctx, float* kernel_ms, int32_t* n_launches);
 
 /* The context's hipStream_t, as an opaque pointer. */
 void* pp_stream(pp_ctx* ctx);
+
+/* Target search of the last pp_propagate (enum pp_search_kind) and, for the
+ * grid, its cell edge (m), cell counts dims[3] and number of cell entries.
+ * The grid replaces the reference's linear scan (src/photon_prop_cuda.jl:
+ * 497-506) with the same first-in-index result. */
+int pp_search_info(pp_ctx* ctx, int32_t* search, float* cell, int32_t* dims, int32_t* n_items);
+
+/* Photon steps, sphere tests and grid-cell visits executed by the last
+ * pp_propagate of a context created with cfg->count_work (roofline
+ * accounting; the hits are identical to the uninstrumented kernel's). */
+int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* cell_visits);
 
 /* ---- step-function evaluation (parity tests) ----------------------------
  * Evaluates one device step function over n explicit inputs on the GPU,

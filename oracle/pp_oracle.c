@@ -72,80 +72,88 @@ static void ppo_block(uint32_t word, uint64_t pid, uint32_t src, const uint32_t 
 
 /* ------------------------------------------------------------------ */
 /* Step functions (pure, explicit uniforms)                            */
+/*                                                                     */
+/* Each function takes the Float32 values the reference holds (inputs, */
+/* uniforms), evaluates its mathematics in double precision and rounds */
+/* its Float32 outputs once, so the oracle sits within 0.5 ulp of the  */
+/* exact value of every step function.                                 */
 /* ------------------------------------------------------------------ */
-#define PPO_TWO_PI_F ((float)(2.0 * 3.14159265358979323846))
+#define PPO_TWO_PI_F ((float)(2.0 * 3.14159265358979323846)) /* T(2*pi) */
 #define PPO_C_VAC_M_NS 0.299792458 /* src/photon_prop_cuda.jl:37 */
 
 /* uniform(min, max) = min + u*(max-min)   src/photon_prop_cuda.jl:48-51 */
 static float ppo_uniform(float lo, float hi, float u) { return lo + u * (hi - lo); }
 
-/* PhysicsTools.sph_to_cart(theta, phi) = (sin t cos p, sin t sin p, cos t) */
-void ppo_sph_to_cart(float theta, float phi, float out[3]) {
-    float st = sinf(theta), ct = cosf(theta);
-    float sp = sinf(phi), cp = cosf(phi);
-    out[0] = cp * st;
-    out[1] = sp * st;
-    out[2] = ct;
+/* phi = uniform(0, T(2pi)) as a Float32 variable */
+static float ppo_phi(float u) { return ppo_uniform(0.0f, PPO_TWO_PI_F, u); }
+
+static void sph_to_cart_d(double ct, double st, double phi, float out[3]) {
+    out[0] = (float)(cos(phi) * st);
+    out[1] = (float)(sin(phi) * st);
+    out[2] = (float)ct;
 }
 
-/* initialize_direction_isotropic   src/photon_prop_cuda.jl:73-77 */
+/* PhysicsTools.sph_to_cart(theta, phi) = (sin t cos p, sin t sin p, cos t) */
+void ppo_sph_to_cart(float theta, float phi, float out[3]) {
+    sph_to_cart_d(cos((double)theta), sin((double)theta), (double)phi, out);
+}
+
+/* initialize_direction_isotropic   src/photon_prop_cuda.jl:73-77:
+ * theta = acos(uniform(-1, 1)), phi = uniform(0, 2pi) */
 void ppo_isotropic_dir(float u1, float u2, float out[3]) {
-    float theta = acosf(ppo_uniform(-1.0f, 1.0f, u1));
-    float phi = ppo_uniform(0.0f, PPO_TWO_PI_F, u2);
-    ppo_sph_to_cart(theta, phi, out);
+    double theta = acos((double)ppo_uniform(-1.0f, 1.0f, u1));
+    sph_to_cart_d(cos(theta), sin(theta), (double)ppo_phi(u2), out);
 }
 
 /* PhysicsTools.rot_from_ez_fast(a, b): rotate b by the rotation that takes
  * e_z to a. Restated as the minimal (Rodrigues) rotation about
  * k = e_z x a / |e_z x a| by theta = angle(e_z, a); a = -e_z turns by pi
  * about x. Convention unpinned (see file header). */
-void ppo_rot_from_ez(const float a[3], const float b[3], float out[3]) {
-    float s = sqrtf(a[0] * a[0] + a[1] * a[1]);
-    float c = a[2];
-    if (s == 0.0f) {
-        if (c > 0.0f) {
-            out[0] = b[0]; out[1] = b[1]; out[2] = b[2];
-        } else {
-            out[0] = b[0]; out[1] = -b[1]; out[2] = -b[2];
-        }
+static void rot_from_ez_d(const double a[3], const double b[3], double out[3]) {
+    double s = sqrt(a[0] * a[0] + a[1] * a[1]);
+    double c = a[2];
+    if (s == 0.0) {
+        out[0] = b[0];
+        out[1] = c > 0.0 ? b[1] : -b[1];
+        out[2] = c > 0.0 ? b[2] : -b[2];
         return;
     }
-    float kx = -a[1] / s, ky = a[0] / s; /* kz = 0 */
-    float kxb0 = ky * b[2];
-    float kxb1 = -(kx * b[2]);
-    float kxb2 = kx * b[1] - ky * b[0];
-    float kdb = kx * b[0] + ky * b[1];
-    float omc = 1.0f - c;
+    double kx = -a[1] / s, ky = a[0] / s; /* kz = 0 */
+    double kxb0 = ky * b[2], kxb1 = -kx * b[2], kxb2 = kx * b[1] - ky * b[0];
+    double kdb = kx * b[0] + ky * b[1];
+    double omc = 1.0 - c;
     out[0] = b[0] * c + kxb0 * s + kx * kdb * omc;
     out[1] = b[1] * c + kxb1 * s + ky * kdb * omc;
     out[2] = b[2] * c + kxb2 * s;
 }
 
-/* CherenkovMediumBase hg_scattering_func(g): inverse CDF of Henyey-Greenstein,
- * clamped to [-1, 1]. */
-float ppo_hg_cos(float g, float u) {
-    float t = (1.0f - g * g) / (1.0f + g * (2.0f * u - 1.0f));
-    float c = (1.0f / (2.0f * g)) * (1.0f + g * g - t * t);
-    if (c < -1.0f) c = -1.0f;
-    if (c > 1.0f) c = 1.0f;
-    return c;
+void ppo_rot_from_ez(const float a[3], const float b[3], float out[3]) {
+    double ad[3] = {a[0], a[1], a[2]}, bd[3] = {b[0], b[1], b[2]}, o[3];
+    rot_from_ez_d(ad, bd, o);
+    out[0] = (float)o[0]; out[1] = (float)o[1]; out[2] = (float)o[2];
 }
+
+static double clamp1d(double x) { return x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x); }
+
+/* CherenkovMediumBase hg_scattering_func(g): inverse CDF of Henyey-Greenstein,
+ * cos = (1 + g^2 - ((1 - g^2)/(1 + g(2u - 1)))^2) / (2g), clamped to [-1, 1]. */
+static double hg_cos_d(double g, double u) {
+    double t = (1.0 - g * g) / (1.0 + g * (2.0 * u - 1.0));
+    return clamp1d((1.0 + g * g - t * t) / (2.0 * g));
+}
+float ppo_hg_cos(float g, float u) { return (float)hg_cos_d(g, u); }
 
 /* Einstein-Smoluchowski phase function p(x) ~ 1 + b x^2 on [-1, 1]: solve
  * CDF(x) = u, i.e. (b/3)x^3 + x + (1+b/3) - u(2+2b/3) = 0, by Cardano's
- * formula for the depressed cubic x^3 + p x + q = 0 (one real root, p > 0).
- * Computed in double, rounded to float, clamped to [-1, 1]. */
-float ppo_es_cos(float b, float u) {
-    double bb = b, uu = u;
-    double p = 3.0 / bb;
-    double q = (3.0 / bb) * ((1.0 + bb / 3.0) - uu * (2.0 + 2.0 * bb / 3.0));
+ * formula for the depressed cubic x^3 + p x + q = 0 (one real root, p > 0),
+ * clamped to [-1, 1]. */
+static double es_cos_d(double b, double u) {
+    double p = 3.0 / b;
+    double q = (3.0 / b) * ((1.0 + b / 3.0) - u * (2.0 + 2.0 * b / 3.0));
     double disc = sqrt(q * q / 4.0 + p * p * p / 27.0);
-    double x = cbrt(-q / 2.0 + disc) + cbrt(-q / 2.0 - disc);
-    float c = (float)x;
-    if (c < -1.0f) c = -1.0f;
-    if (c > 1.0f) c = 1.0f;
-    return c;
+    return clamp1d(cbrt(-q / 2.0 + disc) + cbrt(-q / 2.0 - disc));
 }
+float ppo_es_cos(float b, float u) { return (float)es_cos_d(b, u); }
 
 /* MixedHGES(g, b, hg_fraction): HG with probability hg_fraction, else ES.
  * Draw order: branch choice, then the cosine. */
@@ -154,36 +162,38 @@ float ppo_mixed_hges_cos(float g, float b, float hg_fraction, float u_choice, fl
     return ppo_es_cos(b, u);
 }
 
-/* update_direction   src/photon_prop_cuda.jl:280-298 (cos theta given) */
+/* update_direction   src/photon_prop_cuda.jl:280-298 (cos theta given):
+ * new direction (cos p sin t, sin p sin t, cos t) about e_z, rotated to dir */
 void ppo_update_direction(const float dir[3], float cos_sca_theta, float u_phi, float out[3]) {
-    float sin_sca_theta = sqrtf(1.0f - cos_sca_theta * cos_sca_theta);
-    float sca_phi = ppo_uniform(0.0f, PPO_TWO_PI_F, u_phi);
-    float sin_sca_phi = sinf(sca_phi), cos_sca_phi = cosf(sca_phi);
-    float nd[3] = {cos_sca_phi * sin_sca_theta, sin_sca_phi * sin_sca_theta, cos_sca_theta};
-    ppo_rot_from_ez(dir, nd, out);
+    double ct = cos_sca_theta;
+    double st = sqrt(1.0 - ct * ct);
+    double phi = ppo_phi(u_phi);
+    double nd[3] = {cos(phi) * st, sin(phi) * st, ct};
+    double a[3] = {dir[0], dir[1], dir[2]}, o[3];
+    rot_from_ez_d(a, nd, o);
+    out[0] = (float)o[0]; out[1] = (float)o[1]; out[2] = (float)o[2];
 }
 
 /* _intersect_nsphere + check_intersection(::Spherical)
- *   src/photon_prop_cuda.jl:309-351. Returns 1 and *d on hit. */
+ *   src/photon_prop_cuda.jl:309-351. Near root d = -a - sqrt(b) of the ray,
+ *   hit iff b >= 0 and 0 < d < step. Returns 1 and *d on hit. */
 int ppo_intersect_sphere(const float c[3], float r, const float pos[3], const float dir[3],
                          float step, float* d_out) {
-    float target_rsq = r * r;
-    float dp0 = pos[0] - c[0], dp1 = pos[1] - c[1], dp2 = pos[2] - c[2];
-    float a = dir[0] * dp0 + dir[1] * dp1 + dir[2] * dp2;
-    float pp = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
-    float b = a * a - (pp - target_rsq);
-    int isec = b >= 0.0f;
-    b = fabsf(b);
-    float d = isec ? (-a - sqrtf(b)) : -1.0f;
-    if ((d > 0.0f) && (d < step)) {
-        *d_out = d;
+    double target_rsq = (double)(r * r); /* radius^2 is a Float32 value */
+    double dp0 = (double)pos[0] - c[0], dp1 = (double)pos[1] - c[1], dp2 = (double)pos[2] - c[2];
+    double a = dir[0] * dp0 + dir[1] * dp1 + dir[2] * dp2;
+    double pp = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+    double b = a * a - (pp - target_rsq);
+    double d = b >= 0.0 ? (-a - sqrt(fabs(b))) : -1.0;
+    if ((d > 0.0) && (d < (double)step)) {
+        *d_out = (float)d;
         return 1;
     }
     return 0;
 }
 
 /* step_size = -log(eta) * sca_len   src/photon_prop_cuda.jl:488-490 */
-float ppo_step_length(float eta, float sca_len) { return -logf(eta) * sca_len; }
+float ppo_step_length(float eta, float sca_len) { return (float)(-log((double)eta) * sca_len); }
 
 /* Kopelevich scattering length (CherenkovMediumBase, KopelevichScatteringModel)
  * times sca_scale   src/water_properties.jl:95-100, constants :34-35.
@@ -192,7 +202,7 @@ float ppo_kopelevich_sca_len(float wl, float vs, float vl, float sca_scale) {
     double x = (double)wl / 550.0;
     double b = 0.0017 * pow(x, -4.3) + 1.34 * (double)vs * pow(x, -1.7) +
                0.312 * (double)vl * pow(x, -0.3);
-    return (float)(1.0 / b) * sca_scale;
+    return (float)(1.0 / b * (double)sca_scale);
 }
 
 /* Quan & Fry (1995) phase refractive index of sea water (wl in nm, S permille,
@@ -243,17 +253,16 @@ float ppo_phase_cos(const pp_medium* m, float u_choice, float u) {
 /* sample_cherenkov_track_direction (CherenkovMediumBase): direction of a
  * cascade's emitting track about e_z with the clsim track-angle law
  * p(x) ~ x^(a-1) exp(-b x^a), x = 1 - cos, a = 0.39, b = 2.61, truncated at
- * x = 2 by inverse-CDF sampling. Draw order: cos theta, then phi. */
+ * x = 2 by inverse-CDF sampling with the Float32 normalisation
+ * I = 1 - exp(-b 2^a). Draw order: cos theta, then phi. */
 void ppo_cherenkov_track_dir(float u1, float u2, float out[3]) {
     const float a = 0.39f, b = 2.61f;
-    float I = 1.0f - expf(-b * powf(2.0f, a));
-    float ct = 1.0f - powf(-logf(1.0f - u1 * I) / b, 1.0f / a);
-    if (ct < -1.0f) ct = -1.0f;
-    float st = sqrtf(1.0f - ct * ct);
-    float phi = PPO_TWO_PI_F * u2;
-    out[0] = st * cosf(phi);
-    out[1] = st * sinf(phi);
-    out[2] = ct;
+    const float I = 1.0f - expf(-b * powf(2.0f, a));
+    double ct = 1.0 - pow(-log(1.0 - (double)u1 * I) / b, 1.0 / (double)a);
+    if (ct < -1.0) ct = -1.0;
+    /* cos_theta is a Float32 variable before sin_theta = sqrt(1 - cos^2) */
+    ct = (double)(float)ct;
+    sph_to_cart_d(ct, sqrt(1.0 - ct * ct), (double)(PPO_TWO_PI_F * u2), out);
 }
 
 /* ------------------------------------------------------------------ */
@@ -269,8 +278,8 @@ typedef struct {
 static float ppo_normal(const ppo_stream* s, uint32_t sub) {
     uint32_t r[4];
     ppo_block(PP_CTR_SAMPLER | (sub << 24), s->pid, s->src, s->key, r);
-    float u1 = ppo_u01_open0(r[0]), u2 = ppo_u01(r[1]);
-    return sqrtf(-2.0f * logf(u1)) * cosf(PPO_TWO_PI_F * u2);
+    double u1 = ppo_u01_open0(r[0]), phi = ppo_phi(ppo_u01(r[1]));
+    return (float)(sqrt(-2.0 * log(u1)) * cos(phi));
 }
 
 /* PhysicsTools.rand_gamma(shape, scale): Marsaglia & Tsang (2000). Each
@@ -279,35 +288,34 @@ static float ppo_normal(const ppo_stream* s, uint32_t sub) {
  * sub-stream's boost block. After PP_GAMMA_MAX_BLOCKS blocks without
  * acceptance (probability < 1e-100 for shape >= 0.1) returns d*scale. */
 float ppo_gamma(float shape, float scale, const ppo_stream* s, uint32_t sub) {
-    float boost = 1.0f;
+    double boost = 1.0, k = shape;
     if (shape < 1.0f) {
         uint32_t r[4];
         ppo_block(PP_CTR_SAMPLER | (sub << 24) | PP_SAMPLER_BOOST_BLOCK, s->pid, s->src, s->key,
                   r);
-        boost = powf(ppo_u01_open0(r[0]), 1.0f / shape);
-        shape = shape + 1.0f;
+        boost = pow((double)ppo_u01_open0(r[0]), 1.0 / k);
+        k = k + 1.0;
     }
-    float d = shape - 1.0f / 3.0f;
-    float c = 1.0f / sqrtf(9.0f * d);
+    double d = k - 1.0 / 3.0;
+    double c = 1.0 / sqrt(9.0 * d);
     for (uint32_t i = 0; i < PP_GAMMA_MAX_BLOCKS; ++i) {
         uint32_t r[4];
         ppo_block(PP_CTR_SAMPLER | (sub << 24) | i, s->pid, s->src, s->key, r);
-        float u1 = ppo_u01_open0(r[0]), u2 = ppo_u01(r[1]);
-        float rad = sqrtf(-2.0f * logf(u1));
-        float z[2] = {rad * cosf(PPO_TWO_PI_F * u2), rad * sinf(PPO_TWO_PI_F * u2)};
-        float ua[2] = {ppo_u01_open0(r[2]), ppo_u01_open0(r[3])};
-        for (int k = 0; k < 2; ++k) {
-            float x = z[k];
-            float v = 1.0f + c * x;
-            if (v <= 0.0f) continue;
+        double rad = sqrt(-2.0 * log((double)ppo_u01_open0(r[0])));
+        double phi = ppo_phi(ppo_u01(r[1]));
+        double z[2] = {rad * cos(phi), rad * sin(phi)};
+        double ua[2] = {ppo_u01_open0(r[2]), ppo_u01_open0(r[3])};
+        for (int j = 0; j < 2; ++j) {
+            double x = z[j];
+            double v = 1.0 + c * x;
+            if (v <= 0.0) continue;
             v = v * v * v;
-            float u = ua[k];
-            if (u < 1.0f - 0.0331f * (x * x) * (x * x) ||
-                logf(u) < 0.5f * x * x + d * (1.0f - v + logf(v)))
-                return d * v * scale * boost;
+            double u = ua[j];
+            if (u < 1.0 - 0.0331 * (x * x) * (x * x) || log(u) < 0.5 * x * x + d * (1.0 - v + log(v)))
+                return (float)(d * v * scale * boost);
         }
     }
-    return d * scale * boost;
+    return (float)(d * scale * boost);
 }
 
 /* PhysicsTools.rand_beta(a, b) = X/(X+Y), X ~ Gamma(a), Y ~ Gamma(b) */
@@ -345,10 +353,10 @@ static void ppo_cherenkov_direction(const pp_source* src, float n_phase, const p
                                     int smear, float out[3]) {
     uint32_t x[4];
     ppo_block(PP_CTR_INIT_EXTRA | 1u, s->pid, s->src, s->key, x);
-    float theta_c = acosf(1.0f / n_phase); /* cherenkov_angle, beta = 1 */
-    float phi = ppo_uniform(0.0f, PPO_TWO_PI_F, ppo_u01(x[0]));
+    /* cherenkov_angle = acos(1/n) (beta = 1) */
+    double cos_c = 1.0 / (double)n_phase;
     float ph[3], tmp[3];
-    ppo_sph_to_cart(theta_c, phi, ph);
+    sph_to_cart_d(cos_c, sqrt(1.0 - cos_c * cos_c), (double)ppo_phi(ppo_u01(x[0])), ph);
     if (smear == 1) {
         float track[3];
         ppo_cherenkov_track_dir(ppo_u01(x[1]), ppo_u01(x[2]), track);
@@ -356,9 +364,9 @@ static void ppo_cherenkov_direction(const pp_source* src, float n_phase, const p
         memcpy(ph, tmp, sizeof ph);
     } else if (smear == 2) {
         float tcos = ppo_beta(src->beta_a, src->beta_b, s) * 2.0f - 1.0f;
-        float tphi = ppo_uniform(0.0f, PPO_TWO_PI_F, ppo_u01(x[1]));
         float track[3];
-        ppo_sph_to_cart(acosf(tcos), tphi, track);
+        sph_to_cart_d(tcos, sqrt(1.0 - (double)tcos * tcos), (double)ppo_phi(ppo_u01(x[1])),
+                      track);
         ppo_rot_from_ez(track, ph, tmp);
         memcpy(ph, tmp, sizeof ph);
     }
@@ -397,8 +405,8 @@ void ppo_init_photon(const pp_source* src, const pp_medium* m, const ppo_spectru
     }
     case PP_SRC_COLLIMATED_ISOTROPIC: { /* :186-195 */
         float phi = ppo_uniform(0.0f, PPO_TWO_PI_F, ppo_u01(r[1]));
-        float cos_theta = ppo_uniform(src->cut_angle_cos, 1.0f, ppo_u01(r[2]));
-        ppo_sph_to_cart(acosf(cos_theta), phi, st->dir);
+        double ct = ppo_uniform(src->cut_angle_cos, 1.0f, ppo_u01(r[2]));
+        sph_to_cart_d(ct, sqrt(1.0 - ct * ct), (double)phi, st->dir);
         break;
     }
     case PP_SRC_PENCIL: /* :197-208 */

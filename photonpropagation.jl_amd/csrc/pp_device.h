@@ -24,13 +24,18 @@ constexpr float kCVacMNs = 0.299792458f;          // src/photon_prop_cuda.jl:37
 // the key is bumped by the Weyl constants between rounds. Each 32x32->64
 // product is one v_mad_u64_u32; the key schedule stays in SGPRs.
 // ---------------------------------------------------------------------------
+// 3-input XOR in one v_bitop3_b32 (truth table 0x96 = a ^ b ^ c)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c.x;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c.z;
-        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1,
-                       (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
+        c = make_uint4(xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1,
+                       xor3((uint32_t)(p0 >> 32), c.w, k1), (uint32_t)p0);
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -56,16 +61,59 @@ struct Stream {
 struct f3 {
     float x, y, z;
 };
-__device__ __forceinline__ f3 make3(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ __forceinline__ f3 make3(float x, float y, float z) { return f3{x, y, z}; }
+
+// ---------------------------------------------------------------------------
+// Arithmetic helpers for the FP32 VALU
+// ---------------------------------------------------------------------------
+
+// 1/x: hardware reciprocal (1 ulp) refined by one Newton step (~0.5 ulp),
+// 3 instructions instead of the IEEE division sequence.
+__device__ __forceinline__ float rcp_nr(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+
+// Cube root of x > 0: hardware log2/exp2 estimate, one Newton step.
+__device__ __forceinline__ float cbrt_pos(float x) {
+    const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
+    // y' = y + (x / y^2 - y) / 3
+    return fmaf(fmaf(x, rcp_nr(y * y), -y), 1.0f / 3.0f, y);
+}
+
+// sin(2 pi u), cos(2 pi u) for u in [0, 1): exact reduction to an octant
+// (u*8 is exact), Taylor polynomials on [0, pi/4] (truncation < 2e-9), and
+// octant symmetry. ~25 instructions instead of a general sincosf.
+__device__ __forceinline__ void sincos_2pi(float u, float* s, float* c) {
+    const float v = u * 8.0f;
+    const float fo = floorf(v);
+    const int o = (int)fo & 7;
+    const float f = v - fo;                    // exact
+    const float x = ((o & 1) ? 1.0f - f : f) * 0.785398163397448310f;  // [0, pi/4]
+    const float x2 = x * x;
+    float sn = fmaf(x2, 2.7557319e-6f, -1.9841270e-4f);
+    sn = fmaf(x2, sn, 8.3333333e-3f);
+    sn = fmaf(x2, sn, -1.6666667e-1f);
+    sn = fmaf(x2 * x, sn, x);
+    float cs = fmaf(x2, -2.7557319e-7f, 2.4801587e-5f);
+    cs = fmaf(x2, cs, -1.3888889e-3f);
+    cs = fmaf(x2, cs, 4.1666667e-2f);
+    cs = fmaf(x2, cs, -0.5f);
+    cs = fmaf(x2, cs, 1.0f);
+    const bool swap = ((o + 1) & 2) != 0;
+    const float a = swap ? cs : sn, b = swap ? sn : cs;
+    *s = (o & 4) ? -a : a;
+    *c = ((o + 2) & 4) ? -b : b;
+}
 
 // ---------------------------------------------------------------------------
 // Directions
 // ---------------------------------------------------------------------------
 
-// (sin t cos p, sin t sin p, cos t) from cos t and p = 2*pi*u
+// (sin t cos p, sin t sin p, cos t) from cos t, sin t and p = 2*pi*u
 __device__ __forceinline__ f3 dir_from_cos_phi(float ct, float st, float u_phi) {
     float sp, cp;
-    sincosf(u_phi * kTwoPi, &sp, &cp);
+    sincos_2pi(u_phi, &sp, &cp);
     return make3(cp * st, sp * st, ct);
 }
 
@@ -85,7 +133,7 @@ __device__ __forceinline__ f3 rot_from_ez(f3 a, f3 b) {
     const float c = a.z;
     const float s2 = a.x * a.x + a.y * a.y;
     const bool up = c >= 0.0f;
-    const float h = (up ? 1.0f : (1.0f - c)) / (up ? (1.0f + c) : s2);
+    const float h = (up ? 1.0f : (1.0f - c)) * rcp_nr(up ? (1.0f + c) : s2);
     const float vdb = a.x * b.y - a.y * b.x;
     f3 o;
     o.x = c * b.x + a.x * b.z - h * a.y * vdb;
@@ -125,15 +173,15 @@ __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg
 __device__ __forceinline__ float clamp1(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
 
 __device__ __forceinline__ float hg_cos(const Phase& p, float u) {
-    const float t = p.hg_B / (p.hg_D0 + p.hg_D1 * u);
+    const float t = p.hg_B * rcp_nr(p.hg_D0 + p.hg_D1 * u);
     return clamp1(p.hg_A - p.hg_C * (t * t));
 }
 
 __device__ __forceinline__ float es_cos(const Phase& p, float u) {
     const float q = p.es_Q * (1.0f - 2.0f * u);
     const float hq = 0.5f * fabsf(q);
-    const float w = cbrtf(hq + sqrtf(hq * hq + p.es_P3cube));
-    const float x = w - p.es_P3 / w;
+    const float w = cbrt_pos(hq + sqrtf(hq * hq + p.es_P3cube));
+    const float x = w - p.es_P3 * rcp_nr(w);
     return clamp1(q > 0.0f ? -x : x);
 }
 
@@ -150,14 +198,26 @@ __device__ __forceinline__ f3 update_direction(f3 dir, float ct, float u_phi) {
 
 // ---------------------------------------------------------------------------
 // Ray-sphere test: _intersect_nsphere   src/photon_prop_cuda.jl:309-351
-// Returns the near root d if 0 < d < step, else a value that fails the test.
+// Near root d = -a - sqrt(b), a = dir.dp, b = a^2 - (|dp|^2 - r^2); hit iff
+// b >= 0 and 0 < d < step. b is evaluated as
+//     b = r^2 - |dp - a dir|^2 + a^2 (|dir|^2 - 1),
+// the same value without the cancellation of a^2 against |dp|^2 for a far
+// sphere; the last term keeps it equal to the reference's discriminant for a
+// direction that has drifted from unit length. dir_excess() gives
+// |dir|^2 - 1 once per step (FP64 FMAs: the squares of floats are exact).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool sphere_hit(float4 t, f3 pos, f3 dir, float step, float& d) {
+__host__ __device__ __forceinline__ float dir_excess(f3 dir) {
+    const double x = dir.x, y = dir.y, z = dir.z;
+    return (float)::fma(x, x, ::fma(y, y, ::fma(z, z, -1.0)));
+}
+
+__host__ __device__ __forceinline__ bool sphere_hit(float4 t, f3 pos, f3 dir, float excess,
+                                                    float step, float& d) {
     const float dx = pos.x - t.x, dy = pos.y - t.y, dz = pos.z - t.z;
     const float a = dir.x * dx + dir.y * dy + dir.z * dz;
-    const float pp = dx * dx + dy * dy + dz * dz;
-    const float b = a * a - (pp - t.w);  // t.w = r^2
-    d = (b >= 0.0f) ? (-a - sqrtf(fabsf(b))) : -1.0f;
+    const float qx = dx - a * dir.x, qy = dy - a * dir.y, qz = dz - a * dir.z;
+    const float b = (a * excess) * a + (t.w - (qx * qx + qy * qy + qz * qz));  // t.w = r^2
+    d = (b >= 0.0f) ? (-a - sqrtf(b)) : -1.0f;
     return (d > 0.0f) & (d < step);
 }
 

@@ -21,6 +21,8 @@
 #include <vector>
 
 #include "pp_device.h"
+#include "pp_grid_build.h"
+#include "pp_search.h"
 
 using pp::f3;
 
@@ -45,6 +47,8 @@ struct Counters {
     unsigned long long n_ph_sim;  // completed photons
     unsigned int err;             // 1 = overflow
     unsigned int pad;
+    // work counts of the instrumented kernel (pp_config.count_work)
+    unsigned long long steps, tests, cells;
 };
 
 struct KParams {
@@ -55,6 +59,10 @@ struct KParams {
     uint64_t n_photons;
     const float4* targets;
     int32_t n_targets;
+    // uniform grid over the targets (GridSearch)
+    pp::GridDesc grid;
+    const uint32_t* cell_start;
+    const uint16_t* cell_items;
     int32_t spec_n;
     int32_t spec_map;
     float spec_mono;
@@ -213,12 +221,25 @@ __device__ __forceinline__ bool store_hits(bool has_hit, const HitRec& h, uint32
 // ---------------------------------------------------------------------------
 // propagation kernel: cuda_propagate_photons!   src/photon_prop_cuda.jl:429-549
 // ---------------------------------------------------------------------------
-template <int PHASE>
-__global__ void __launch_bounds__(1024) propagate_brute(KParams p) {
+using pp::lds_bytes;
+
+enum { kSearchBrute = 0, kSearchGrid = 1 };
+
+// COUNT: instrumented variant that also counts photon steps, sphere tests
+// and grid-cell visits (roofline accounting); the results are identical.
+template <int PHASE, int SEARCH, bool COUNT = false>
+__global__ void __launch_bounds__(1024) propagate_kernel(KParams p) {
     extern __shared__ float4 lds[];
     float4* s_tgt = lds;
-    float* s_spec = reinterpret_cast<float*>(lds + p.n_targets);
+    uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + p.n_targets);
+    uint16_t* s_items = reinterpret_cast<uint16_t*>(s_start + (SEARCH == kSearchGrid ? p.grid.n_cells + 1 : 0));
+    float* s_spec = reinterpret_cast<float*>(
+        s_items + (SEARCH == kSearchGrid ? ((p.grid.n_items + 1) & ~1) : 0));
     for (int i = threadIdx.x; i < p.n_targets; i += blockDim.x) s_tgt[i] = p.targets[i];
+    if (SEARCH == kSearchGrid) {
+        for (int i = threadIdx.x; i <= p.grid.n_cells; i += blockDim.x) s_start[i] = p.cell_start[i];
+        for (int i = threadIdx.x; i < p.grid.n_items; i += blockDim.x) s_items[i] = p.cell_items[i];
+    }
     for (int i = threadIdx.x; i < p.spec_n; i += blockDim.x) s_spec[i] = p.spec[i];
     __syncthreads();
 
@@ -227,7 +248,15 @@ __global__ void __launch_bounds__(1024) propagate_brute(KParams p) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long n_done = 0;
-    const int nt = p.n_targets;
+    typename std::conditional<SEARCH == kSearchGrid, pp::GridSearch, pp::BruteSearch>::type search;
+    if constexpr (SEARCH == kSearchGrid) {
+        search = pp::GridSearch{s_tgt, s_start, s_items, p.grid};
+    } else {
+        search = pp::BruteSearch{s_tgt, p.n_targets};
+    }
+
+    pp::Work work;
+    unsigned long long n_step_done = 0;
 
     for (; pid < p.n_photons; pid += stride) {
         pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), p.src_ix};
@@ -242,15 +271,9 @@ __global__ void __launch_bounds__(1024) propagate_brute(KParams p) {
             const uint4 r = s.block((uint32_t)step);
             const float step_size = -logf(pp::u01_open0(r.x)) * sca_len;
             // first target in index order whose near root lies inside the step
-            int found = -1;
             float dt = 0.0f;
-            for (int t = 0; t < nt; ++t) {
-                float d;
-                if (pp::sphere_hit(s_tgt[t], pos, dir, step_size, d) && found < 0) {
-                    found = t;
-                    dt = d;
-                }
-            }
+            const int found = search.template find<COUNT>(pos, dir, step_size, dt, work);
+            if (COUNT) ++n_step_done;
             if (found >= 0) {
                 pos = add_scaled(pos, dt, dir);
                 dist += dt;
@@ -275,6 +298,11 @@ __global__ void __launch_bounds__(1024) propagate_brute(KParams p) {
         ++n_done;
     }
     if (n_done) atomicAdd(&p.counters->n_ph_sim, n_done);
+    if (COUNT) {
+        atomicAdd(&p.counters->steps, n_step_done);
+        atomicAdd(&p.counters->tests, (unsigned long long)work.tests);
+        atomicAdd(&p.counters->cells, (unsigned long long)work.cells);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -328,8 +356,9 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
     }
     case PP_FN_SPHERE_ISEC: {
         float d;
+        const pp::f3 dir = pp::make3(x[7], x[8], x[9]);
         const bool h = pp::sphere_hit(make_float4(x[0], x[1], x[2], x[3] * x[3]),
-                                      pp::make3(x[4], x[5], x[6]), pp::make3(x[7], x[8], x[9]),
+                                      pp::make3(x[4], x[5], x[6]), dir, pp::dir_excess(dir),
                                       x[10], d);
         y[0] = h ? 1.0f : 0.0f;
         y[1] = h ? d : 0.0f;
@@ -433,7 +462,17 @@ struct pp_ctx {
     int n_targets = -1;
     float4* d_targets = nullptr;
     int tgt_cap = 0;
+    std::vector<float4> h_targets;  // x, y, z, r^2
+    std::vector<float> h_radius;
+    // uniform grid over the targets, rebuilt when targets or cell size change
+    pp::GridDesc grid{};
+    float grid_cell_built = -1.0f;  // requested cell size of the built grid
+    uint32_t* d_cell_start = nullptr;
+    uint16_t* d_cell_items = nullptr;
+    size_t cell_cap = 0, item_cap = 0;
+    int last_search = 0;
     int64_t n_hits = 0, n_ph_sim = 0;
+    int64_t work_steps = 0, work_tests = 0, work_cells = 0;
     float last_ms = 0.0f;
     int32_t n_launch = 0;
     bool perm_valid = false;
@@ -442,12 +481,47 @@ struct pp_ctx {
 
 namespace {
 
+// Make the context's device current and drop any error state left by an
+// earlier, already reported call, so that launch checks see only their own.
 int set_dev(pp_ctx* c) {
     PP_HIP(hipSetDevice(c->device));
+    (void)hipGetLastError();
     return PP_OK;
 }
 
 constexpr int kMaxLdsBytes = 160 * 1024;
+constexpr size_t kGridLdsBudget = 64 * 1024;  // grid tables + targets + spectrum per block
+
+// (Re)build the target grid for the requested cell edge and upload it.
+int build_grid(pp_ctx* c, float cell_req) {
+    std::vector<float> xyz(3 * (size_t)c->n_targets);
+    for (int i = 0; i < c->n_targets; ++i) {
+        xyz[3 * i] = c->h_targets[i].x;
+        xyz[3 * i + 1] = c->h_targets[i].y;
+        xyz[3 * i + 2] = c->h_targets[i].z;
+    }
+    pp::GridTables t;
+    if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, c->spec_n,
+                               kGridLdsBudget, t))
+        return fail(PP_ERR_INVALID, "grid: no cell size fits the LDS budget");
+    if (t.start.size() > c->cell_cap) {
+        if (c->d_cell_start) PP_HIP(hipFree(c->d_cell_start));
+        c->d_cell_start = nullptr;
+        PP_HIP(hipMalloc(&c->d_cell_start, t.start.size() * 4));
+        c->cell_cap = t.start.size();
+    }
+    if (t.items.size() > c->item_cap) {
+        if (c->d_cell_items) PP_HIP(hipFree(c->d_cell_items));
+        c->d_cell_items = nullptr;
+        PP_HIP(hipMalloc(&c->d_cell_items, t.items.size() * 2));
+        c->item_cap = t.items.size();
+    }
+    PP_HIP(hipMemcpy(c->d_cell_start, t.start.data(), t.start.size() * 4, hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(c->d_cell_items, t.items.data(), t.items.size() * 2, hipMemcpyHostToDevice));
+    c->grid = t.g;
+    c->grid_cell_built = cell_req;
+    return PP_OK;
+}
 
 }  // namespace
 
@@ -508,26 +582,29 @@ int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* c
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
         return cleanup(fail(PP_ERR_HIP, "stream/event creation failed"));
     const size_t cap = (size_t)hit_capacity;
-    const size_t bytes = cap * (size_t)pp_hit_record_bytes() + 256 * 8;
+    // column layout: 12 float columns, module, n_steps, photon id, source;
+    // every column starts on a 256-byte boundary
+    const size_t col_bytes[kNumFloatCols + 4] = {
+        cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4,
+        cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 8, cap * 4};
+    size_t col_off[kNumFloatCols + 4];
+    size_t bytes = 0;
+    for (int i = 0; i < kNumFloatCols + 4; ++i) {
+        col_off[i] = bytes;
+        bytes += (col_bytes[i] + 255) & ~(size_t)255;
+    }
     if (hipMalloc(&ctx->hit_mem, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return cleanup(fail(PP_ERR_NOMEM, "pp_create: cannot allocate " + std::to_string(bytes) +
                                               " bytes of hit buffer"));
     }
     char* base = (char*)ctx->hit_mem;
-    size_t off = 0;
-    auto take = [&](size_t n) {
-        char* p = base + off;
-        off += (n + 255) & ~(size_t)255;
-        return p;
-    };
-    for (int i = 0; i < kNumFloatCols; ++i) ctx->hits.f[i] = (float*)take(cap * 4);
-    ctx->hits.module_ix = (int32_t*)take(cap * 4);
-    ctx->hits.n_steps = (int32_t*)take(cap * 4);
-    ctx->hits.photon_id = (uint64_t*)take(cap * 8);
-    ctx->hits.source_ix = (int32_t*)take(cap * 4);
+    for (int i = 0; i < kNumFloatCols; ++i) ctx->hits.f[i] = (float*)(base + col_off[i]);
+    ctx->hits.module_ix = (int32_t*)(base + col_off[kNumFloatCols]);
+    ctx->hits.n_steps = (int32_t*)(base + col_off[kNumFloatCols + 1]);
+    ctx->hits.photon_id = (uint64_t*)(base + col_off[kNumFloatCols + 2]);
+    ctx->hits.source_ix = (int32_t*)(base + col_off[kNumFloatCols + 3]);
     ctx->hits.capacity = cap;
-    if (off > bytes + 255 * 20) return cleanup(fail(PP_ERR_STATE, "internal layout error"));
     if (hipMalloc(&ctx->d_counters, sizeof(Counters)) != hipSuccess)
         return cleanup(fail(PP_ERR_NOMEM, "pp_create: counters"));
     *out = ctx;
@@ -542,6 +619,8 @@ int pp_destroy(pp_ctx* ctx) {
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_spec) (void)hipFree(ctx->d_spec);
     if (ctx->d_targets) (void)hipFree(ctx->d_targets);
+    if (ctx->d_cell_start) (void)hipFree(ctx->d_cell_start);
+    if (ctx->d_cell_items) (void)hipFree(ctx->d_cell_items);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -627,6 +706,10 @@ int pp_set_targets(pp_ctx* ctx, const pp_target* t, int n) {
         PP_HIP(hipMemcpy(ctx->d_targets, h.data(), (size_t)n * sizeof(float4),
                          hipMemcpyHostToDevice));
     ctx->n_targets = n;
+    ctx->h_targets.assign(h.begin(), h.begin() + n);
+    ctx->h_radius.resize((size_t)n);
+    for (int i = 0; i < n; ++i) ctx->h_radius[(size_t)i] = t[i].radius;
+    ctx->grid_cell_built = -1.0f;  // grid is rebuilt on the next propagation
     return PP_OK;
 }
 
@@ -663,9 +746,35 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     if (int rc = set_dev(ctx)) return rc;
 
     const int block = ctx->cfg.block_size;
-    const size_t lds = (size_t)ctx->n_targets * sizeof(float4) + (size_t)ctx->spec_n * sizeof(float);
-    auto kern = ctx->medium.phase.kind == PP_PHASE_HG ? propagate_brute<PP_PHASE_HG>
-                                                     : propagate_brute<PP_PHASE_MIXED_HG_ES>;
+    // search: grid for more than a handful of targets unless asked otherwise
+    int search = ctx->cfg.search;
+    if (search == PP_SEARCH_AUTO) search = ctx->n_targets > 8 ? PP_SEARCH_GRID : PP_SEARCH_BRUTE;
+    if (ctx->n_targets == 0) search = PP_SEARCH_BRUTE;
+    const bool grid = search == PP_SEARCH_GRID;
+    if (grid) {
+        const float want = ctx->cfg.grid_cell;
+        if (ctx->grid_cell_built != want || ctx->grid_cell_built < -0.5f)
+            if (int rc = build_grid(ctx, want)) return rc;
+    }
+    const size_t lds = lds_bytes(ctx->n_targets, grid ? ctx->grid.n_cells : 0,
+                                 grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
+    if (lds > (size_t)kMaxLdsBytes)
+        return fail(PP_ERR_INVALID, "pp_propagate: targets and tables exceed the LDS budget");
+    const bool hg = ctx->medium.phase.kind == PP_PHASE_HG;
+    const bool count = ctx->cfg.count_work != 0;
+    using KernelFn = void (*)(KParams);
+    // [count][grid][hg]
+    static const KernelFn kKernels[2][2][2] = {
+        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, false>,
+          propagate_kernel<PP_PHASE_HG, kSearchBrute, false>},
+         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, false>,
+          propagate_kernel<PP_PHASE_HG, kSearchGrid, false>}},
+        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, true>,
+          propagate_kernel<PP_PHASE_HG, kSearchBrute, true>},
+         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, true>,
+          propagate_kernel<PP_PHASE_HG, kSearchGrid, true>}}};
+    const KernelFn kern = kKernels[count][grid][hg];
+    ctx->last_search = search;
     int occ = 0;
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds));
     if (occ < 1) return fail(PP_ERR_INVALID, "pp_propagate: kernel does not fit one block per CU");
@@ -683,6 +792,11 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     kp.n_steps = n_steps;
     kp.targets = ctx->d_targets;
     kp.n_targets = ctx->n_targets;
+    if (grid) {
+        kp.grid = ctx->grid;
+        kp.cell_start = ctx->d_cell_start;
+        kp.cell_items = ctx->d_cell_items;
+    }
     kp.spec_n = ctx->spec_n;
     kp.spec_map = ctx->spec_map;
     kp.spec_mono = ctx->spec_mono;
@@ -710,6 +824,9 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     PP_HIP(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
     ctx->n_hits = (int64_t)std::min<unsigned long long>(hc.n_hits, ctx->hits.capacity);
     ctx->n_ph_sim = (int64_t)hc.n_ph_sim;
+    ctx->work_steps = (int64_t)hc.steps;
+    ctx->work_tests = (int64_t)hc.tests;
+    ctx->work_cells = (int64_t)hc.cells;
     ctx->perm_valid = false;
     if (n_hits_out) *n_hits_out = ctx->n_hits;
     if (n_ph_sim_out) *n_ph_sim_out = ctx->n_ph_sim;
@@ -725,6 +842,30 @@ int pp_last_timing(pp_ctx* ctx, float* kernel_ms, int32_t* n_launches) {
 }
 
 void* pp_stream(pp_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* cell_visits) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_work_counts: NULL ctx");
+    if (!ctx->cfg.count_work)
+        return fail(PP_ERR_STATE, "pp_work_counts: context created without count_work");
+    if (steps) *steps = ctx->work_steps;
+    if (sphere_tests) *sphere_tests = ctx->work_tests;
+    if (cell_visits) *cell_visits = ctx->work_cells;
+    return PP_OK;
+}
+
+int pp_search_info(pp_ctx* ctx, int32_t* search, float* cell, int32_t* dims, int32_t* n_items) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_search_info: NULL ctx");
+    if (search) *search = ctx->last_search;
+    const bool g = ctx->last_search == PP_SEARCH_GRID && ctx->grid_cell_built > -0.5f;
+    if (cell) *cell = g ? ctx->grid.h : 0.0f;
+    if (dims) {
+        dims[0] = g ? ctx->grid.nx : 0;
+        dims[1] = g ? ctx->grid.ny : 0;
+        dims[2] = g ? ctx->grid.nz : 0;
+    }
+    if (n_items) *n_items = g ? ctx->grid.n_items : 0;
+    return PP_OK;
+}
 
 int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3,
                  float* init_dir3, float* wavelength, void* time, int time_f64,

@@ -1,0 +1,154 @@
+// pp_search.h - target search of one photon step.
+//
+// Semantics of src/photon_prop_cuda.jl:494-506: among the targets whose near
+// root d satisfies 0 < d < step, report the one with the smallest index
+// ("first target in index order", not the nearest).
+//
+// BruteSearch tests every target (wave-uniform loop over LDS, broadcast
+// reads) - the reference's own algorithm.
+// GridSearch walks the step segment through a uniform grid of cubic cells
+// (3-D DDA, Amanatides & Woo) whose cells list the targets overlapping them,
+// so only targets near the segment are tested. Every target whose hit point
+// lies on the segment overlaps a visited cell (cells are binned with a
+// margin far above float rounding of the walk), and the smallest hit index
+// over all visited candidates is kept, so the result equals BruteSearch's.
+#pragma once
+
+#include "pp_device.h"
+
+namespace pp {
+
+struct GridDesc {
+    float ox, oy, oz;  // lower corner
+    float h, inv_h;    // cell edge
+    int nx, ny, nz;
+    int n_cells, n_items;
+};
+
+// Work done by one search (instrumented builds only).
+struct Work {
+    uint32_t tests = 0, cells = 0;
+};
+
+struct BruteSearch {
+    const float4* tgt;
+    int n;
+    __host__ __device__ __forceinline__ int operator()(f3 pos, f3 dir, float step, float& dt) const {
+        Work w;
+        return find<false>(pos, dir, step, dt, w);
+    }
+    template <bool COUNT>
+    __host__ __device__ __forceinline__ int find(f3 pos, f3 dir, float step, float& dt,
+                                                 Work& w) const {
+        if (COUNT) w.tests += (uint32_t)n;
+        const float ex = dir_excess(dir);
+        int found = -1;
+        for (int t = 0; t < n; ++t) {
+            float d;
+            if (sphere_hit(tgt[t], pos, dir, ex, step, d) && found < 0) {
+                found = t;
+                dt = d;
+            }
+        }
+        return found;
+    }
+};
+
+__host__ __device__ __forceinline__ float nonzero(float x) {
+    return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x;
+}
+
+// Reciprocal for the grid walk: the hardware estimate on the device (the walk
+// only needs its boundary parameters to be consistent, not correctly
+// rounded); a plain division when the same code is compiled for the host.
+__host__ __device__ __forceinline__ float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+__host__ __device__ __forceinline__ int clampi(int v, int hi) {
+    return v < 0 ? 0 : (v > hi ? hi : v);
+}
+
+struct GridSearch {
+    const float4* tgt;
+    const uint32_t* cell_start;  // n_cells + 1
+    const uint16_t* items;       // target indices per cell
+    GridDesc g;
+
+    __host__ __device__ __forceinline__ int operator()(f3 pos, f3 dir, float step, float& dt) const {
+        Work w;
+        return find<false>(pos, dir, step, dt, w);
+    }
+    template <bool COUNT>
+    __host__ __device__ __forceinline__ int find(f3 pos, f3 dir, float step, float& dt,
+                                                 Work& w) const {
+        // direction components pushed away from zero; the walk's step signs
+        // below come from these same values so that they agree with ix_..iz_
+        const float ix_ = fast_rcp(nonzero(dir.x)), iy_ = fast_rcp(nonzero(dir.y)),
+                    iz_ = fast_rcp(nonzero(dir.z));
+        // clip the segment [0, step] to the grid box
+        const float ex = g.ox + g.nx * g.h, ey = g.oy + g.ny * g.h, ez = g.oz + g.nz * g.h;
+        const float tx0 = (g.ox - pos.x) * ix_, tx1 = (ex - pos.x) * ix_;
+        const float ty0 = (g.oy - pos.y) * iy_, ty1 = (ey - pos.y) * iy_;
+        const float tz0 = (g.oz - pos.z) * iz_, tz1 = (ez - pos.z) * iz_;
+        const float t_in = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float t_out = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), step));
+        int found = -1;
+        if (!(t_in <= t_out)) return found;
+        const float exc = dir_excess(dir);
+        // entry cell
+        const float px = pos.x + dir.x * t_in, py = pos.y + dir.y * t_in, pz = pos.z + dir.z * t_in;
+        int cx = clampi((int)floorf((px - g.ox) * g.inv_h), g.nx - 1);
+        int cy = clampi((int)floorf((py - g.oy) * g.inv_h), g.ny - 1);
+        int cz = clampi((int)floorf((pz - g.oz) * g.inv_h), g.nz - 1);
+        const int sx = ix_ > 0.0f ? 1 : -1, sy = iy_ > 0.0f ? 1 : -1, sz = iz_ > 0.0f ? 1 : -1;
+        // ray parameter of the next cell boundary on each axis
+        float tmx = (g.ox + (float)(cx + (sx > 0)) * g.h - pos.x) * ix_;
+        float tmy = (g.oy + (float)(cy + (sy > 0)) * g.h - pos.y) * iy_;
+        float tmz = (g.oz + (float)(cz + (sz > 0)) * g.h - pos.z) * iz_;
+        const float tdx = g.h * fabsf(ix_), tdy = g.h * fabsf(iy_), tdz = g.h * fabsf(iz_);
+        // cells left before leaving the grid along each axis, cell index and
+        // its increments: the walk advances with selects, not branches
+        int rx = sx > 0 ? g.nx - 1 - cx : cx;
+        int ry = sy > 0 ? g.ny - 1 - cy : cy;
+        int rz = sz > 0 ? g.nz - 1 - cz : cz;
+        const int dcx = sx, dcy = sy * g.nx, dcz = sz * g.nx * g.ny;
+        int c = (cz * g.ny + cy) * g.nx + cx;
+        const int max_visits = g.nx + g.ny + g.nz;
+        for (int v = 0; v < max_visits; ++v) {
+            const uint32_t b = cell_start[c], e = cell_start[c + 1];
+            if (COUNT) {
+                w.cells += 1;
+                w.tests += e - b;
+            }
+            for (uint32_t k = b; k < e; ++k) {
+                const int t = items[k];
+                float d;
+                if (sphere_hit(tgt[t], pos, dir, exc, step, d) && (found < 0 || t < found)) {
+                    found = t;
+                    dt = d;
+                }
+            }
+            const float tn = fminf(fminf(tmx, tmy), tmz);
+            const bool ax = tn == tmx;
+            const bool ay = !ax && tn == tmy;
+            const bool az = !ax && !ay;
+            const int rem = ax ? rx : (ay ? ry : rz);
+            if (tn >= t_out || rem == 0) break;
+            c += ax ? dcx : (ay ? dcy : dcz);
+            rx -= ax;
+            ry -= ay;
+            rz -= az;
+            tmx += ax ? tdx : 0.0f;
+            tmy += ay ? tdy : 0.0f;
+            tmz += az ? tdz : 0.0f;
+        }
+        return found;
+    }
+};
+
+}  // namespace pp

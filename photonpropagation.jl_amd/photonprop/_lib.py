@@ -79,7 +79,7 @@ class pp_target(Structure):
 class pp_config(Structure):
     _fields_ = [("block_size", c_int32), ("waves_per_simd", c_int32), ("search", c_int32),
                 ("grid_cell", c_float), ("deterministic_order", c_int32),
-                ("reserved", c_int32 * 11)]
+                ("count_work", c_int32), ("reserved", c_int32 * 10)]
 
 
 assert ctypes.sizeof(pp_source) == 96
@@ -106,6 +106,9 @@ SIGNATURES = [
                              c_void_p]),
     ("pp_last_timing", c_int, [c_void_p, POINTER(c_float), POINTER(c_int32)]),
     ("pp_stream", c_void_p, [c_void_p]),
+    ("pp_search_info", c_int, [c_void_p, POINTER(c_int32), POINTER(c_float), POINTER(c_int32),
+                               POINTER(c_int32)]),
+    ("pp_work_counts", c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
     ("pp_eval_fn", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int64]),
 ]
 

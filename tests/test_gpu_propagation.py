@@ -27,7 +27,14 @@ def oracle_run(P, O, wl, photon_scaling=1.0, capacity=1 << 22):
                        photon_scaling=photon_scaling, capacity=capacity, **kw)
 
 
-def compare_photon_by_photon(g, r, max_diff_frac=0.005):
+def compare_photon_by_photon(g, r, max_diff_frac=0.005, tail_frac=0.02):
+    """Same photons hit the same module at the same step; emission values
+    agree to round-off for all of them. Transport values agree to round-off
+    (5 mm + 2e-5 of the path) for all but `tail_frac` of them: HG(0.95)
+    puts ~4e-4 of the scatterings per step within 1e-6 of cos = 1, where a
+    1-ulp difference of the Float32 cosine becomes a >1e-5 rad direction
+    difference that grows along the remaining path; those photons still
+    agree within the module's diameter."""
     gk = g["source_ix"].astype(np.uint64) << np.uint64(40) | g["photon_id"]
     rk = r["source_ix"].astype(np.uint64) << np.uint64(40) | r["photon_id"]
     assert np.all(np.diff(gk.astype(np.float64)) > 0), "GPU hits not in (source, photon) order"
@@ -38,10 +45,15 @@ def compare_photon_by_photon(g, r, max_diff_frac=0.005):
     assert np.all(g["n_steps"][gi] == r["n_steps"][ri])
     np.testing.assert_allclose(g["wavelength"][gi], r["wavelength"][ri], rtol=2e-6)
     np.testing.assert_allclose(g["initial_direction"][gi], r["init_dir"][ri], atol=2e-5)
-    np.testing.assert_allclose(g["position"][gi], r["pos"][ri], atol=5e-3)
-    np.testing.assert_allclose(g["direction"][gi], r["dir"][ri], atol=5e-3)
-    np.testing.assert_allclose(g["time"][gi], r["time"][ri], rtol=1e-4, atol=1e-3)
-    np.testing.assert_allclose(g["dist_travelled"][gi], r["dist"][ri], rtol=1e-4, atol=1e-3)
+    dpos = np.max(np.abs(g["position"][gi] - r["pos"][ri]), axis=1)
+    ddir = np.max(np.abs(g["direction"][gi] - r["dir"][ri]), axis=1)
+    dt = np.abs(g["time"][gi] - r["time"][ri]) / np.maximum(r["time"][ri], 1.0)
+    dd = np.abs(g["dist_travelled"][gi] - r["dist"][ri]) / np.maximum(r["dist"][ri], 1.0)
+    tight = (dpos < 5e-3 + 2e-5 * r["dist"][ri]) & (ddir < 5e-3) & (dt < 1e-4) & (dd < 1e-4)
+    print(f"tail {np.mean(~tight):.4f}; max dpos {dpos.max():.3g} m, ddir {ddir.max():.3g}, "
+          f"dt {dt.max():.3g}, dd {dd.max():.3g}")
+    assert np.mean(~tight) <= tail_frac, np.mean(~tight)
+    assert np.all(dpos < 1.0) and np.all(dt < 1e-2) and np.all(dd < 1e-2)
     return len(common), n_diff
 
 
@@ -203,3 +215,47 @@ def test_statistical_parity_cascade(P, O, dev):
     p = stats.ks_2samp(g["position"][:, 2], r["pos"][:, 2]).pvalue
     assert p > 0.01
     del mods
+
+
+@pytest.mark.parametrize("cell", [0.0, 4.0, 30.0])
+def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell):
+    """The uniform-grid search returns the reference's first-in-index target:
+    hits identical bit for bit to the brute-force scan."""
+    import ctypes
+    wl = P.workloads.cascade_16_strings(photons=3_000_000, seed=11)
+    out = {}
+    for search in (1, 2):
+        cfg = P._lib.pp_config()
+        cfg.search = search
+        cfg.grid_cell = cell
+        cfg.deterministic_order = 1
+        _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
+        with d:
+            out[search] = gpu_run(P, d, wl)
+            info = ctypes.c_int32()
+            h = ctypes.c_float()
+            dims = (ctypes.c_int32 * 3)()
+            ni = ctypes.c_int32()
+            gpu_lib.pp_search_info(d.handle, ctypes.byref(info), ctypes.byref(h), dims,
+                                   ctypes.byref(ni))
+            assert info.value == search
+            if search == 2:
+                print(f"grid cell {h.value:.2f} m, dims {list(dims)}, {ni.value} entries")
+    (gb, nb), (gg, ng) = out[1], out[2]
+    assert nb == ng and len(gb["wavelength"]) > 300
+    for k in gb:
+        assert gb[k].tobytes() == gg[k].tobytes(), k
+
+
+def test_grid_search_1000_modules_matches_oracle(P, O, dev):
+    """Uneven multi-source batch over 1000 modules (grid search by default)
+    photon by photon against the oracle's brute-force scan."""
+    wl = P.workloads.uneven_batch_1000(n_sources=8, seed=3, min_photons=1e5, max_photons=3e5)
+    g, gsim = gpu_run(P, dev, wl)
+    r, rsim = oracle_run(P, O, wl)
+    assert gsim == rsim
+    # long track and far-source paths: more photons cross a module edge after
+    # a round-off direction difference (see compare_photon_by_photon)
+    n, d = compare_photon_by_photon(g, r, max_diff_frac=0.02)
+    print(f"1000 modules, 6 sources: {n} common hits, {d} differing")
+    assert n > 20

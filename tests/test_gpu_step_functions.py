@@ -1,8 +1,11 @@
 """GPU step functions (pp_eval_fn, the kernel's own inlined code) against the
 CPU oracle on identical inputs.
 
-Tolerance: 4 FP32 ulp of the output's scale. Unit-vector components are
-measured at scale 1 (ulp(1) = 2^-23); scalars at their own magnitude.
+The oracle evaluates each function exactly (double precision, one rounding
+of the Float32 output). Tolerance: 4 FP32 ulp of the output's scale plus the
+function's own conditioning, i.e. the largest change of the exact output
+when any one Float32 input moves by 1 ulp (kappa). Unit-vector components
+are measured at scale 1 (ulp(1) = 2^-23); scalars at their magnitude.
 Philox and the spectrum lookup are bit-exact."""
 import numpy as np
 import pytest
@@ -33,6 +36,33 @@ def both(P, O, dev, fn, x, **kw):
     return g, r
 
 
+def kappa(O, fn, x, **kw):
+    """Largest change of the oracle output when one input moves by 1 ulp."""
+    x = np.asarray(x, dtype=np.float32)
+    base = O.eval_fn(fn, x, **kw).astype(np.float64)
+    k = np.zeros_like(base)
+    for j in range(x.shape[1]):
+        for to in (np.float32(np.inf), np.float32(-np.inf)):
+            xp = x.copy()
+            xp[:, j] = np.nextafter(x[:, j], to)
+            k = np.maximum(k, np.abs(O.eval_fn(fn, xp, **kw).astype(np.float64) - base))
+    return k
+
+
+def check_vec(P, O, dev, fn, x, name, scale=None, **kw):
+    """Assert |gpu - oracle| <= 4 ulp(scale) + kappa; print the error in ulp."""
+    g, r = both(P, O, dev, fn, x, **kw)
+    k = kappa(O, fn, x, **kw)
+    u = ULP1 if scale is None else ulp(scale)
+    err = np.abs(g - r) / u
+    excess = (np.abs(g - r) - k) / u
+    report(name, err)
+    print(f"{name}: max over kappa {np.max(excess):.2f} ulp, "
+          f"within 4 ulp without kappa {np.mean(err <= 4):.6f}")
+    assert np.max(excess) <= 4
+    return g, r
+
+
 def test_philox_bit_exact(P, O, dev):
     w = RNG.integers(0, 2**32, (100_000, 6), dtype=np.uint64).astype(np.uint32)
     g = P._lib.eval_fn(dev, "PHILOX", w.view(np.float32)).view(np.uint32)
@@ -42,10 +72,7 @@ def test_philox_bit_exact(P, O, dev):
 
 def test_isotropic_dir(P, O, dev):
     x = RNG.random((200_000, 2), dtype=np.float32)
-    g, r = both(P, O, dev, "ISOTROPIC_DIR", x)
-    err = np.abs(g - r) / ULP1
-    report("isotropic_dir", err)
-    assert err.max() <= 4
+    check_vec(P, O, dev, "ISOTROPIC_DIR", x, "isotropic_dir")
 
 
 def test_rot_from_ez(P, O, dev):
@@ -53,40 +80,28 @@ def test_rot_from_ez(P, O, dev):
     x = np.concatenate([unit(n), unit(n)], axis=1)
     x[:10, :3] = [0, 0, 1]
     x[10:20, :3] = [0, 0, -1]
-    g, r = both(P, O, dev, "ROT_FROM_EZ", x)
-    err = np.abs(g - r) / ULP1
-    report("rot_from_ez", err)
-    assert err.max() <= 4
+    check_vec(P, O, dev, "ROT_FROM_EZ", x, "rot_from_ez")
 
 
 @pytest.mark.parametrize("g_", [0.5, 0.95, 0.99])
 def test_hg_cos(P, O, dev, g_):
     u = RNG.random(200_000, dtype=np.float32)
     x = np.stack([np.full_like(u, g_), u], 1)
-    g, r = both(P, O, dev, "HG_COS", x)
-    err = np.abs(g - r) / ULP1  # cosines: scale 1
-    report(f"hg_cos g={g_}", err)
-    assert err.max() <= 4
+    check_vec(P, O, dev, "HG_COS", x, f"hg_cos g={g_}")
 
 
 @pytest.mark.parametrize("b", [0.835, 0.3])
 def test_es_cos(P, O, dev, b):
     u = RNG.random(200_000, dtype=np.float32)
     x = np.stack([np.full_like(u, b), u], 1)
-    g, r = both(P, O, dev, "ES_COS", x)
-    err = np.abs(g - r) / ULP1
-    report(f"es_cos b={b}", err)
-    assert err.max() <= 4
+    check_vec(P, O, dev, "ES_COS", x, f"es_cos b={b}")
 
 
 def test_update_direction(P, O, dev):
     n = 200_000
     x = np.concatenate([unit(n), RNG.uniform(-1, 1, (n, 1)).astype(np.float32),
                         RNG.random((n, 1), dtype=np.float32)], 1)
-    g, r = both(P, O, dev, "UPDATE_DIR", x)
-    err = np.abs(g - r) / ULP1
-    report("update_direction", err)
-    assert err.max() <= 4
+    check_vec(P, O, dev, "UPDATE_DIR", x, "update_direction")
 
 
 def test_sphere_intersection(P, O, dev):
@@ -99,22 +114,29 @@ def test_sphere_intersection(P, O, dev):
     step = RNG.exponential(40.0, n).astype(np.float32)
     x = np.concatenate([c, r[:, None], p, d, step[:, None]], 1)
     g, ref = both(P, O, dev, "SPHERE_ISEC", x)
-    # exact geometry in float64 to locate ill-conditioned rows (grazing rays,
-    # roots at the step boundary)
+    k = kappa(O, "SPHERE_ISEC", x)
+    # exact geometry: discriminant b = r^2 - |dp - a dir|^2, near root
     pd, cd, dd = p.astype(np.float64), c.astype(np.float64), d.astype(np.float64)
-    a = np.sum(dd * (pd - cd), 1)
-    disc = a * a - (np.sum((pd - cd) ** 2, 1) - r.astype(np.float64) ** 2)
-    root = -a - np.sqrt(np.abs(disc))
-    edge = (np.abs(disc) < 1e-3 * r**2) | (np.abs(root - step) < 1e-4 * step) | (
-        np.abs(root) < 1e-4)
-    same = g[:, 0] == ref[:, 0]
-    assert np.all(same | edge)
+    dp = pd - cd
+    a = np.sum(dd * dp, 1)
+    q = dp - a[:, None] * dd
+    b = r.astype(np.float64) ** 2 - np.sum(q * q, 1)
+    root = -a - np.sqrt(np.abs(b))
+    ndp = np.linalg.norm(dp, axis=1)
+    eps = 16 * 2.0**-24
+    # decisions may differ only for rays within 16 float ulps of a boundary:
+    # grazing (b ~ 0), root at the step end or at the start point
+    edge = (k[:, 0] > 0) | (np.abs(b) < eps * (2 * ndp * np.linalg.norm(q, axis=1) + r**2)) | (
+        np.abs(root - step) < eps * (np.abs(a) + step + ndp)) | (np.abs(root) < eps * ndp)
+    print(f"sphere: {edge.mean():.2e} of rows within the decision band")
+    assert np.all((g[:, 0] == ref[:, 0]) | edge)
     h = (g[:, 0] == 1) & (ref[:, 0] == 1) & ~edge
     assert h.sum() > 10_000
-    # d = -a - sqrt(b): 4 ulp of the larger operand |a|
     err = np.abs(g[h, 1] - ref[h, 1]) / ulp(np.abs(a[h]))
-    report("sphere d", err)
-    assert err.max() <= 4
+    excess = (np.abs(g[h, 1] - ref[h, 1]) - k[h, 1]) / ulp(np.abs(a[h]))
+    report("sphere d (ulp of |a|)", err)
+    print(f"sphere d: max over kappa {np.max(excess):.2f} ulp")
+    assert np.max(excess) <= 4
 
 
 def test_step_length(P, O, dev):
@@ -140,11 +162,25 @@ def test_cascadia_medium_values(P, O, dev):
 
 
 def test_cherenkov_track_direction(P, O, dev):
+    """cos(theta) is a Float32 variable before sin = sqrt(1 - cos^2) (as in
+    the reference): one ulp of that stored cosine is added to kappa, since
+    next to cos = 1 it moves sin(theta) by far more than 4 ulp."""
     x = RNG.random((200_000, 2), dtype=np.float32)
     g, r = both(P, O, dev, "TRACK_DIR", x)
+    k = kappa(O, "TRACK_DIR", x)
+    ct = r[:, 2]
+    st = np.sqrt(np.maximum(1 - ct * ct, 0))
+    for to in (np.inf, -np.inf):
+        c2 = np.nextafter(ct.astype(np.float32), np.float32(to)).astype(np.float64)
+        ds = np.abs(np.sqrt(np.maximum(1 - c2 * c2, 0)) - st)
+        k[:, 0] = np.maximum(k[:, 0], ds + np.abs(c2 - ct))
+        k[:, 1] = np.maximum(k[:, 1], ds + np.abs(c2 - ct))
+        k[:, 2] = np.maximum(k[:, 2], np.abs(c2 - ct))
     err = np.abs(g - r) / ULP1
+    excess = (np.abs(g - r) - k) / ULP1
     report("track_dir", err)
-    assert err.max() <= 4
+    print(f"track_dir: max over kappa {np.max(excess):.2f} ulp")
+    assert np.max(excess) <= 4
 
 
 def test_gamma_sampler_same_draws(P, O, dev):

@@ -1,0 +1,36 @@
+"""The target grid checked on the CPU: the device search code (pp_search.h,
+compiled for the host) under AddressSanitizer must agree with the brute-force
+first-in-index search on every segment, for several scenes and cell sizes."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+from conftest import ROOT
+
+SRC = os.path.join(ROOT, "tests", "hostcheck", "grid_check.cpp")
+
+
+@pytest.fixture(scope="module")
+def grid_check():
+    d = tempfile.mkdtemp()
+    exe = os.path.join(d, "grid_check")
+    cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-host-only", "-O1", "-g",
+           "-Xarch_host", "-fsanitize=address", "-fno-omit-frame-pointer",
+           "-I", os.path.join(ROOT, "include"), "-o", exe, SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+@pytest.mark.parametrize("scene,cell", [(0, 0.0), (0, 3.0), (0, 25.0), (1, 0.0), (1, 0.7),
+                                        (2, 0.0), (2, 12.0)])
+def test_grid_equals_brute_force(grid_check, scene, cell):
+    r = subprocess.run([grid_check, str(scene), "17", "200000", str(cell)], capture_output=True,
+                       text=True, env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
+    rays, hits, mism, nx, ny, nz, n_items = map(int, r.stdout.split()[-7:])
+    print(f"scene {scene} cell {cell}: grid {nx}x{ny}x{nz}, {n_items} entries, {hits} hits")
+    assert hits > 1000
+    assert mism == 0
