@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--cell", type=float, default=0.0, help="grid cell edge in m (0 = auto)")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = library default)")
     ap.add_argument("--wps", type=int, default=0, help="waves per SIMD for grid sizing (0 = auto)")
+    ap.add_argument("--workload", choices=["cascade", "uneven"], default="cascade",
+                    help="cascade: 1e8-photon 10 TeV cascade in 320 modules; uneven: 256 "
+                         "sources of 1e4-1e7 photons in 1000 modules")
+    ap.add_argument("--batch", choices=["auto", "per-source", "persistent", "graph"],
+                    default="auto", help="source scheduling (pp_batch_mode)")
+    ap.add_argument("--chunk", type=int, default=0, help="photons per work chunk (0 = auto)")
     ap.add_argument("--cpu-sample", type=float, default=0.0,
                     help="photons for the CPU baseline (0 = calibrate to ~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -106,8 +112,14 @@ def main():
     if lib.pp_device_count() < 1:
         raise SystemExit("no HIP device visible")
     nph = int(args.photons)
-    wl = P.workloads.cascade_16_strings(photons=nph)
+    if args.workload == "cascade":
+        wl = P.workloads.cascade_16_strings(photons=nph)
+    else:
+        wl = P.workloads.uneven_batch_1000()
+        nph = int(sum(s.photons for s in wl.sources))
     cfg = P._lib.pp_config()
+    cfg.batch_mode = {"auto": 0, "per-source": 1, "persistent": 2, "graph": 3}[args.batch]
+    cfg.chunk_photons = args.chunk
     cfg.search = {"auto": 0, "brute": 1, "grid": 2}[args.search]
     cfg.grid_cell = args.cell
     cfg.block_size = args.block
@@ -120,7 +132,7 @@ def main():
     n_sim = ctypes.c_int64()
 
     def step():
-        rc = lib.pp_propagate(dev.handle, pods, 1, ctypes.c_uint64(wl.seed), wl.n_steps, 1.0,
+        rc = lib.pp_propagate(dev.handle, pods, len(pods), ctypes.c_uint64(wl.seed), wl.n_steps, 1.0,
                               ctypes.byref(n_hits), ctypes.byref(n_sim))
         P._lib.check(rc, "pp_propagate")
 
@@ -148,10 +160,12 @@ def main():
     ccfg.grid_cell = cfg.grid_cell
     ccfg.block_size = cfg.block_size
     ccfg.waves_per_simd = cfg.waves_per_simd
+    ccfg.batch_mode = cfg.batch_mode
+    ccfg.chunk_photons = cfg.chunk_photons
     ccfg.count_work = 1
     _, cdev = P.make_hit_buffers(capacity=1 << 24, config=ccfg)
     P.processing.stage(cdev, wl.targets, wl.medium, wl.spectrum)
-    P._lib.check(lib.pp_propagate(cdev.handle, pods, 1, ctypes.c_uint64(wl.seed), wl.n_steps,
+    P._lib.check(lib.pp_propagate(cdev.handle, pods, len(pods), ctypes.c_uint64(wl.seed), wl.n_steps,
                                   1.0, ctypes.byref(n_hits), ctypes.byref(n_sim)),
                  "pp_propagate (count_work)")
     w_steps, w_tests, w_cells = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
@@ -180,10 +194,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cascade_16_strings", "photons": nph, "targets": n_t,
+        "config": {"workload": wl.name, "photons": nph, "sources": len(pods), "targets": n_t,
                    "n_steps": wl.n_steps, "search": args.search, "cell_m": args.cell,
-                   "block": args.block, "waves_per_simd": args.wps,
-                   "source": "ExtendedCherenkovEmitter 10 TeV e-, 4x4x20 spheres r=0.3 m"},
+                   "block": args.block, "waves_per_simd": args.wps, "batch": args.batch,
+                   "chunk": args.chunk,
+                   "source": ("ExtendedCherenkovEmitter 10 TeV e-, 4x4x20 spheres r=0.3 m"
+                              if args.workload == "cascade" else
+                              "256 cascades/100 m tracks, log-uniform 1e4-1e7 photons, "
+                              "10x10x10 spheres r=0.3 m")},
+        "launches_per_step": int(dev.last_timing()[1]),
         "photon_steps_per_s": steps / (ms_per_step / 1e3),
         "kernel_ms": kms,
         "hits_per_step": int(len(hits["n_steps"])),
@@ -201,7 +220,7 @@ def main():
                               "instrumented kernel"},
     }
     dev.close()
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "cascade":
         out["cpu_baseline"] = cpu_baseline(
             lambda n: P.workloads.cascade_16_strings(photons=n), args.cpu_sample)
     else:

@@ -123,6 +123,24 @@ enum pp_spectrum_mapping {
 /* ---- configuration ---------------------------------------------------- */
 enum pp_search_kind { PP_SEARCH_AUTO = 0, PP_SEARCH_BRUTE = 1, PP_SEARCH_GRID = 2 };
 
+/* How pp_propagate schedules a source list. The reference launches one
+ * kernel per source (src/photon_prop_cuda.jl:925-944). All modes give the
+ * same hits (same photon ids and random streams).
+ *   PP_BATCH_PER_SOURCE  one launch per source
+ *   PP_BATCH_PERSISTENT  one launch for the whole list: waves take chunks of
+ *                        cfg.chunk_photons photons of one source from a
+ *                        global counter (work stealing)
+ *   PP_BATCH_GRAPH       PP_BATCH_PERSISTENT as a hipGraph (counter clear +
+ *                        propagation), instantiated once and re-parameterised
+ *                        on later calls
+ *   PP_BATCH_AUTO        PP_BATCH_PERSISTENT */
+enum pp_batch_mode {
+    PP_BATCH_AUTO = 0,
+    PP_BATCH_PER_SOURCE = 1,
+    PP_BATCH_PERSISTENT = 2,
+    PP_BATCH_GRAPH = 3
+};
+
 typedef struct pp_config {
     int32_t block_size;         /* threads per block, multiple of 64; 0 = 256 */
     int32_t waves_per_simd;     /* grid sizing: blocks = CUs*4*wps*64/block; 0 = auto */
@@ -131,7 +149,10 @@ typedef struct pp_config {
     int32_t deterministic_order;/* pp_copy_hits sorts by (source, photon id)   */
     int32_t count_work;         /* instrumented kernel: counts steps, sphere
                                    tests and grid-cell visits (pp_work_counts) */
-    int32_t reserved[10];
+    int32_t batch_mode;         /* enum pp_batch_mode                          */
+    int32_t chunk_photons;      /* photons per work chunk (multiple of 64);
+                                   0 = auto                                    */
+    int32_t reserved[8];
 } pp_config; /* 64 bytes */
 
 /* ---- random stream layout (the reproducibility contract) ---------------

@@ -1,37 +1,48 @@
 // pp_grid_build.h - host construction of the uniform target grid searched by
-// GridSearch (pp_search.h): cell edge choice, binning, per-cell index lists.
+// GridSearch (pp_search.h): cell edge choice, binning, per-cell index lists
+// and per-cell bounding boxes of the members.
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "pp_search.h"
 
 namespace pp {
 
-// LDS layout of the propagation kernel: targets float4[n_t] | cell starts
-// u32[n_cells+1] | cell items u16[n_items, padded to even] | spectrum float[n_spec]
+// LDS layout of the propagation kernel: targets float4[n_t] | cell records
+// float4[2 n_cells] | cell items u16[n_items, padded to even] | spectrum
+// float[n_spec]
 __host__ __device__ inline size_t lds_bytes(int n_t, int n_cells, int n_items, int n_spec,
                                             bool grid) {
     size_t b = (size_t)n_t * 16;
-    if (grid) b += (size_t)(n_cells + 1) * 4 + (size_t)((n_items + 1) & ~1) * 2;
+    if (grid) b += (size_t)n_cells * 32 + (size_t)((n_items + 1) & ~1) * 2;
     return b + (size_t)n_spec * 4;
 }
 
 struct GridTables {
     GridDesc g{};
-    std::vector<uint32_t> start;  // n_cells + 1
+    std::vector<float4> cells;    // 2 per cell: (lo xyz, start), (hi xyz, count)
     std::vector<uint16_t> items;  // at least one entry
 };
 
+inline float u32_as_float(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 // Bin every target (centre xyz, radius) into the cells its bounding box plus
 // a margin m = 1e-3 h + 1 mm overlaps (the margin is far above the float
-// rounding of the DDA walk). cell_req <= 0 picks an edge giving about 8 cells
-// per target; the edge grows by 1.25x until the tables fit `budget` bytes of
-// LDS together with the targets and n_spec spectrum knots. Each cell lists its
-// targets in ascending index order. Returns false if nothing fits.
+// rounding of the DDA walk). Each cell lists its targets in ascending index
+// order and keeps the union of their boxes, grown by a further 1 cm + 1e-4 h,
+// so that a float slab test of a segment against it can only err towards
+// testing. cell_req <= 0 picks an edge giving about one cell per target; the
+// edge grows by 1.25x until the tables fit `budget` bytes of LDS together
+// with the targets and n_spec spectrum knots. Returns false if nothing fits.
 inline bool build_grid_tables(const float* xyz, const float* radius, int n, float cell_req,
                               int n_spec, size_t budget, GridTables& out) {
     if (n <= 0 || n > 65535) return false;
@@ -44,7 +55,7 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
     float ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = std::max(hi[a] - lo[a], 1e-3f);
     float h = cell_req;
-    if (!(h > 0.0f)) h = std::cbrt(ext[0] * ext[1] * ext[2] / (8.0f * (float)n));
+    if (!(h > 0.0f)) h = std::cbrt(ext[0] * ext[1] * ext[2] / (float)n);
     h = std::max(h, 1e-3f);
     for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f) {
         const float m = 1e-3f * h + 1e-3f;
@@ -55,7 +66,7 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
             dim[a] = std::max(1, (int)std::ceil((ext[a] + 4.0f * m) / h));
         }
         const int64_t n_cells = (int64_t)dim[0] * dim[1] * dim[2];
-        if (n_cells * 4 > (int64_t)budget) continue;
+        if (n_cells * 32 > (int64_t)budget) continue;
         auto range = [&](int i, int a, int& c0, int& c1) {
             const float p = xyz[3 * i + a], r = radius[i];
             c0 = std::max(0, (int)std::floor((p - r - m - o[a]) / h));
@@ -76,11 +87,13 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
                     }
         }
         if (lds_bytes(n, (int)n_cells, (int)n_items, n_spec, true) > budget) continue;
-        out.start.assign((size_t)n_cells + 1, 0);
+        std::vector<uint32_t> start((size_t)n_cells + 1, 0);
         for (int64_t k = 0; k < n_cells; ++k)
-            out.start[(size_t)k + 1] = out.start[(size_t)k] + count[(size_t)k];
-        std::vector<uint32_t> fill(out.start.begin(), out.start.end() - 1);
+            start[(size_t)k + 1] = start[(size_t)k] + count[(size_t)k];
+        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
         out.items.assign((size_t)std::max<int64_t>(n_items, 1), 0);
+        std::vector<float> box_lo(3 * (size_t)n_cells, INFINITY), box_hi(3 * (size_t)n_cells, -INFINITY);
+        const float mb = m + 1e-2f + 1e-4f * h;
         for (int i = 0; i < n; ++i) {
             int x0, x1, y0, y1, z0, z1;
             range(i, 0, x0, x1);
@@ -88,8 +101,22 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
             range(i, 2, z0, z1);
             for (int z = z0; z <= z1; ++z)
                 for (int y = y0; y <= y1; ++y)
-                    for (int x = x0; x <= x1; ++x)
-                        out.items[fill[(size_t)((z * dim[1] + y) * dim[0] + x)]++] = (uint16_t)i;
+                    for (int x = x0; x <= x1; ++x) {
+                        const size_t c = (size_t)((z * dim[1] + y) * dim[0] + x);
+                        out.items[fill[c]++] = (uint16_t)i;
+                        for (int a = 0; a < 3; ++a) {
+                            box_lo[3 * c + a] = std::min(box_lo[3 * c + a], xyz[3 * i + a] - radius[i] - mb);
+                            box_hi[3 * c + a] = std::max(box_hi[3 * c + a], xyz[3 * i + a] + radius[i] + mb);
+                        }
+                    }
+        }
+        out.cells.assign(2 * (size_t)n_cells, float4{});
+        for (int64_t k = 0; k < n_cells; ++k) {
+            const size_t c = (size_t)k;
+            out.cells[2 * c] = make_float4(box_lo[3 * c], box_lo[3 * c + 1], box_lo[3 * c + 2],
+                                           u32_as_float(start[c]));
+            out.cells[2 * c + 1] = make_float4(box_hi[3 * c], box_hi[3 * c + 1], box_hi[3 * c + 2],
+                                               u32_as_float(count[c]));
         }
         GridDesc& g = out.g;
         g.ox = o[0];

@@ -49,6 +49,7 @@ struct Counters {
     unsigned int pad;
     // work counts of the instrumented kernel (pp_config.count_work)
     unsigned long long steps, tests, cells;
+    unsigned long long next_chunk;  // work-stealing counter of the batch kernel
 };
 
 struct KParams {
@@ -61,7 +62,7 @@ struct KParams {
     int32_t n_targets;
     // uniform grid over the targets (GridSearch)
     pp::GridDesc grid;
-    const uint32_t* cell_start;
+    const float4* cells;
     const uint16_t* cell_items;
     int32_t spec_n;
     int32_t spec_map;
@@ -102,9 +103,8 @@ __device__ __forceinline__ f3 cherenkov_dir(const pp_source& src, const pp::Medi
     return pp::rot_from_ez(axis, ph);
 }
 
-__device__ __forceinline__ PhotonInit init_photon(const KParams& p, const float* spec,
-                                                  const pp::Stream& s) {
-    const pp_source& src = p.src;
+__device__ __forceinline__ PhotonInit init_photon(const KParams& p, const pp_source& src,
+                                                  const float* spec, const pp::Stream& s) {
     const uint4 r = s.block(PP_CTR_INIT);
     PhotonInit st;
     st.wl = pp::spectrum_sample(spec, p.spec_n, p.spec_map, p.spec_mono, pp::u01(r.x));
@@ -225,84 +225,188 @@ using pp::lds_bytes;
 
 enum { kSearchBrute = 0, kSearchGrid = 1 };
 
+// LDS tables of one workgroup: targets, grid cells and items, spectrum knots
+struct Lds {
+    float4* tgt;
+    float4* cells;
+    uint16_t* items;
+    float* spec;
+};
+
+template <int SEARCH>
+__device__ __forceinline__ Lds stage_lds(const KParams& p) {
+    extern __shared__ float4 lds[];
+    Lds l;
+    l.tgt = lds;
+    l.cells = lds + p.n_targets;
+    l.items = reinterpret_cast<uint16_t*>(l.cells + (SEARCH == kSearchGrid ? 2 * p.grid.n_cells : 0));
+    l.spec = reinterpret_cast<float*>(l.items + (SEARCH == kSearchGrid ? ((p.grid.n_items + 1) & ~1) : 0));
+    for (int i = threadIdx.x; i < p.n_targets; i += blockDim.x) l.tgt[i] = p.targets[i];
+    if (SEARCH == kSearchGrid) {
+        for (int i = threadIdx.x; i < 2 * p.grid.n_cells; i += blockDim.x) l.cells[i] = p.cells[i];
+        for (int i = threadIdx.x; i < p.grid.n_items; i += blockDim.x) l.items[i] = p.cell_items[i];
+    }
+    for (int i = threadIdx.x; i < p.spec_n; i += blockDim.x) l.spec[i] = p.spec[i];
+    __syncthreads();
+    return l;
+}
+
+template <int SEARCH>
+using SearchT = typename std::conditional<SEARCH == kSearchGrid, pp::GridSearch, pp::BruteSearch>::type;
+
+template <int SEARCH>
+__device__ __forceinline__ SearchT<SEARCH> make_search(const KParams& p, const Lds& l) {
+    if constexpr (SEARCH == kSearchGrid) {
+        return pp::GridSearch{l.tgt, l.cells, l.items, p.grid};
+    } else {
+        return pp::BruteSearch{l.tgt, p.n_targets};
+    }
+}
+
+// One photon's life: emission, then up to n_steps steps of
+// src/photon_prop_cuda.jl:486-539. Returns true on a hit; h holds the record.
+template <bool COUNT, typename Search>
+__device__ __forceinline__ bool transport_photon(const KParams& p, const pp_source& src,
+                                                 uint32_t src_ix, uint64_t pid, const float* spec,
+                                                 const Search& search, const pp::Phase& phase,
+                                                 pp::Work& work, unsigned long long& n_steps_done,
+                                                 HitRec& h) {
+    pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), src_ix};
+    const PhotonInit st = init_photon(p, src, spec, s);
+    f3 pos = st.pos, dir = st.dir;
+    float time = st.time, dist = 0.0f;
+    const float sca_len = st.mv.sca_len, inv_c = st.mv.inv_c_grp;
+    bool hit = false;
+    int32_t module_ix = 0, n_at = 0;
+    for (int32_t step = 1; step <= p.n_steps; ++step) {
+        const uint4 r = s.block((uint32_t)step);
+        const float step_size = -logf(pp::u01_open0(r.x)) * sca_len;
+        // first target in index order whose near root lies inside the step
+        float dt = 0.0f;
+        const int found = search.template find<COUNT>(pos, dir, step_size, dt, work);
+        if (COUNT) ++n_steps_done;
+        if (found >= 0) {
+            pos = add_scaled(pos, dt, dir);
+            dist += dt;
+            time += dt * inv_c;
+            hit = true;
+            module_ix = found + 1;
+            n_at = step;
+            break;
+        }
+        pos = add_scaled(pos, step_size, dir);
+        dist += step_size;
+        time += step_size * inv_c;
+        const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
+        dir = pp::update_direction(dir, ct, pp::u01(r.w));
+    }
+    h = HitRec{pos, dir, st.dir, st.wl, time, dist, module_ix, n_at, pid};
+    return hit;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_counts(const KParams& p, unsigned long long n_done,
+                                             unsigned long long n_steps_done, const pp::Work& work) {
+    if (n_done) atomicAdd(&p.counters->n_ph_sim, n_done);
+    if (COUNT) {
+        atomicAdd(&p.counters->steps, n_steps_done);
+        atomicAdd(&p.counters->tests, (unsigned long long)work.tests);
+        atomicAdd(&p.counters->cells, (unsigned long long)work.cells);
+    }
+}
+
+// Propagation of one source: cuda_propagate_photons! (src/photon_prop_cuda.jl:
+// 429-549) with photons grid-strided over the launch.
 // COUNT: instrumented variant that also counts photon steps, sphere tests
 // and grid-cell visits (roofline accounting); the results are identical.
 template <int PHASE, int SEARCH, bool COUNT = false>
 __global__ void __launch_bounds__(1024) propagate_kernel(KParams p) {
-    extern __shared__ float4 lds[];
-    float4* s_tgt = lds;
-    uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + p.n_targets);
-    uint16_t* s_items = reinterpret_cast<uint16_t*>(s_start + (SEARCH == kSearchGrid ? p.grid.n_cells + 1 : 0));
-    float* s_spec = reinterpret_cast<float*>(
-        s_items + (SEARCH == kSearchGrid ? ((p.grid.n_items + 1) & ~1) : 0));
-    for (int i = threadIdx.x; i < p.n_targets; i += blockDim.x) s_tgt[i] = p.targets[i];
-    if (SEARCH == kSearchGrid) {
-        for (int i = threadIdx.x; i <= p.grid.n_cells; i += blockDim.x) s_start[i] = p.cell_start[i];
-        for (int i = threadIdx.x; i < p.grid.n_items; i += blockDim.x) s_items[i] = p.cell_items[i];
-    }
-    for (int i = threadIdx.x; i < p.spec_n; i += blockDim.x) s_spec[i] = p.spec[i];
-    __syncthreads();
-
-    pp::Medium med = p.med;
-    med.phase.kind = PHASE;
+    const Lds l = stage_lds<SEARCH>(p);
+    pp::Phase phase = p.med.phase;
+    phase.kind = PHASE;
+    const auto search = make_search<SEARCH>(p, l);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long n_done = 0;
-    typename std::conditional<SEARCH == kSearchGrid, pp::GridSearch, pp::BruteSearch>::type search;
-    if constexpr (SEARCH == kSearchGrid) {
-        search = pp::GridSearch{s_tgt, s_start, s_items, p.grid};
-    } else {
-        search = pp::BruteSearch{s_tgt, p.n_targets};
-    }
-
+    unsigned long long n_done = 0, n_steps_done = 0;
     pp::Work work;
-    unsigned long long n_step_done = 0;
-
-    for (; pid < p.n_photons; pid += stride) {
-        pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), p.src_ix};
-        const PhotonInit st = init_photon(p, s_spec, s);
-        f3 pos = st.pos, dir = st.dir;
-        float time = st.time, dist = 0.0f;
-        const float sca_len = st.mv.sca_len, inv_c = st.mv.inv_c_grp;
-        bool hit = false;
-        int32_t module_ix = 0, n_at = 0;
-
-        for (int32_t step = 1; step <= p.n_steps; ++step) {
-            const uint4 r = s.block((uint32_t)step);
-            const float step_size = -logf(pp::u01_open0(r.x)) * sca_len;
-            // first target in index order whose near root lies inside the step
-            float dt = 0.0f;
-            const int found = search.template find<COUNT>(pos, dir, step_size, dt, work);
-            if (COUNT) ++n_step_done;
-            if (found >= 0) {
-                pos = add_scaled(pos, dt, dir);
-                dist += dt;
-                time += dt * inv_c;
-                hit = true;
-                module_ix = found + 1;
-                n_at = step;
-                break;
-            }
-            pos = add_scaled(pos, step_size, dir);
-            dist += step_size;
-            time += step_size * inv_c;
-            const float ct = pp::phase_cos(med.phase, pp::u01(r.y), pp::u01(r.z));
-            dir = pp::update_direction(dir, ct, pp::u01(r.w));
-        }
-
-        HitRec h{pos, dir, st.dir, st.wl, time, dist, module_ix, n_at, pid};
+    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < p.n_photons;
+         pid += stride) {
+        HitRec h;
+        const bool hit = transport_photon<COUNT>(p, p.src, p.src_ix, pid, l.spec, search, phase,
+                                                 work, n_steps_done, h);
         if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
             atomicOr(&p.counters->err, 1u);
             break;
         }
         ++n_done;
     }
-    if (n_done) atomicAdd(&p.counters->n_ph_sim, n_done);
-    if (COUNT) {
-        atomicAdd(&p.counters->steps, n_step_done);
-        atomicAdd(&p.counters->tests, (unsigned long long)work.tests);
-        atomicAdd(&p.counters->cells, (unsigned long long)work.cells);
+    flush_counts<COUNT>(p, n_done, n_steps_done, work);
+}
+
+// Batch of sources in one persistent launch. Work items are chunks of
+// `chunk` consecutive photons of one source (chunk_prefix[s] = first chunk
+// of source s). Each wave takes the next chunk from a global counter
+// (work stealing: waves that drew cheap photons simply take more chunks)
+// and exits when the counter passes the last chunk, so every wave ends.
+// Photon ids and random streams are those of the one-launch-per-source
+// kernel: the hits are identical.
+struct BatchParams {
+    const pp_source* sources;
+    const uint64_t* n_photons;     // per source
+    const uint64_t* chunk_prefix;  // n_sources + 1
+    int32_t n_sources;
+    uint32_t chunk;                // multiple of 64
+};
+
+template <int PHASE, int SEARCH, bool COUNT = false>
+__global__ void __launch_bounds__(1024) propagate_batch_kernel(KParams p, BatchParams b) {
+    const Lds l = stage_lds<SEARCH>(p);
+    pp::Phase phase = p.med.phase;
+    phase.kind = PHASE;
+    const auto search = make_search<SEARCH>(p, l);
+    const uint64_t total = b.chunk_prefix[b.n_sources];
+    const uint32_t lane = __lane_id();
+    unsigned long long n_done = 0, n_steps_done = 0;
+    pp::Work work;
+    bool overflow = false;
+    for (;;) {
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(&p.counters->next_chunk, 1ull);
+        const uint64_t ck = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(got >> 32)) << 32) |
+                            __builtin_amdgcn_readfirstlane((uint32_t)got);
+        if (ck >= total) break;
+        // source of this chunk: last s with chunk_prefix[s] <= ck
+        int lo = 0, hi = b.n_sources - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (b.chunk_prefix[mid] <= ck) lo = mid;
+            else hi = mid - 1;
+        }
+        const int si = lo;
+        const pp_source src = b.sources[si];
+        const uint64_t n_src = b.n_photons[si];
+        const uint64_t first = (ck - b.chunk_prefix[si]) * b.chunk;
+        const uint64_t end = first + b.chunk < n_src ? first + b.chunk : n_src;
+        for (uint64_t base = first; base < end; base += 64) {
+            const uint64_t pid = base + lane;
+            if (pid >= end || overflow) continue;
+            HitRec h;
+            const bool hit = transport_photon<COUNT>(p, src, (uint32_t)si, pid, l.spec, search,
+                                                     phase, work, n_steps_done, h);
+            if (!store_hits(hit, h, (uint32_t)si, p.hits, p.counters)) {
+                atomicOr(&p.counters->err, 1u);
+                overflow = true;
+                continue;
+            }
+            ++n_done;
+        }
+        if (__ballot(overflow)) break;  // the call fails; stop taking chunks
     }
+    flush_counts<COUNT>(p, n_done, n_steps_done, work);
+}
+
+// Zeroes the counters in a captured graph (a kernel node, not a memset node).
+__global__ void clear_counters_kernel(Counters* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = Counters{};
 }
 
 // ---------------------------------------------------------------------------
@@ -467,7 +571,7 @@ struct pp_ctx {
     // uniform grid over the targets, rebuilt when targets or cell size change
     pp::GridDesc grid{};
     float grid_cell_built = -1.0f;  // requested cell size of the built grid
-    uint32_t* d_cell_start = nullptr;
+    float4* d_cells = nullptr;
     uint16_t* d_cell_items = nullptr;
     size_t cell_cap = 0, item_cap = 0;
     int last_search = 0;
@@ -477,6 +581,18 @@ struct pp_ctx {
     int32_t n_launch = 0;
     bool perm_valid = false;
     std::vector<int64_t> perm;
+    // source table of the persistent batch kernel: device copy and pinned
+    // host staging, laid out sources | photon counts | chunk prefix
+    void* d_batch = nullptr;
+    void* h_batch = nullptr;
+    size_t batch_cap = 0;  // bytes of each
+    // hipGraph of the batch mode PP_BATCH_GRAPH: counter clear -> propagation
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraphNode_t graph_prop = nullptr;
+    const void* graph_fn = nullptr;
+    unsigned graph_blocks = 0, graph_block = 0;
+    size_t graph_lds = 0;
 };
 
 namespace {
@@ -504,11 +620,11 @@ int build_grid(pp_ctx* c, float cell_req) {
     if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, c->spec_n,
                                kGridLdsBudget, t))
         return fail(PP_ERR_INVALID, "grid: no cell size fits the LDS budget");
-    if (t.start.size() > c->cell_cap) {
-        if (c->d_cell_start) PP_HIP(hipFree(c->d_cell_start));
-        c->d_cell_start = nullptr;
-        PP_HIP(hipMalloc(&c->d_cell_start, t.start.size() * 4));
-        c->cell_cap = t.start.size();
+    if (t.cells.size() > c->cell_cap) {
+        if (c->d_cells) PP_HIP(hipFree(c->d_cells));
+        c->d_cells = nullptr;
+        PP_HIP(hipMalloc(&c->d_cells, t.cells.size() * sizeof(float4)));
+        c->cell_cap = t.cells.size();
     }
     if (t.items.size() > c->item_cap) {
         if (c->d_cell_items) PP_HIP(hipFree(c->d_cell_items));
@@ -516,10 +632,182 @@ int build_grid(pp_ctx* c, float cell_req) {
         PP_HIP(hipMalloc(&c->d_cell_items, t.items.size() * 2));
         c->item_cap = t.items.size();
     }
-    PP_HIP(hipMemcpy(c->d_cell_start, t.start.data(), t.start.size() * 4, hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(c->d_cells, t.cells.data(), t.cells.size() * sizeof(float4),
+                     hipMemcpyHostToDevice));
     PP_HIP(hipMemcpy(c->d_cell_items, t.items.data(), t.items.size() * 2, hipMemcpyHostToDevice));
     c->grid = t.g;
     c->grid_cell_built = cell_req;
+    return PP_OK;
+}
+
+// Blocks that fill the device: occupancy limit or cfg.waves_per_simd.
+int max_resident_blocks(pp_ctx* c, const void* fn, int block, size_t lds, int64_t& out) {
+    int occ = 0;
+    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
+    if (occ < 1) return fail(PP_ERR_INVALID, "pp_propagate: kernel does not fit one block per CU");
+    out = (int64_t)occ * c->n_cu;
+    if (c->cfg.waves_per_simd > 0)
+        out = std::max<int64_t>(1, (int64_t)c->n_cu * 4 * c->cfg.waves_per_simd * 64 / block);
+    return PP_OK;
+}
+
+// One launch per source, as src/photon_prop_cuda.jl:925-944.
+int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
+                      const std::vector<uint64_t>& nph, bool grid, bool hg, bool count, size_t lds) {
+    using KernelFn = void (*)(KParams);
+    // [count][grid][hg]
+    static const KernelFn kKernels[2][2][2] = {
+        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, false>,
+          propagate_kernel<PP_PHASE_HG, kSearchBrute, false>},
+         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, false>,
+          propagate_kernel<PP_PHASE_HG, kSearchGrid, false>}},
+        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, true>,
+          propagate_kernel<PP_PHASE_HG, kSearchBrute, true>},
+         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, true>,
+          propagate_kernel<PP_PHASE_HG, kSearchGrid, true>}}};
+    const KernelFn kern = kKernels[count][grid][hg];
+    const int block = c->cfg.block_size;
+    int64_t max_blocks = 0;
+    if (int rc = max_resident_blocks(c, (const void*)kern, block, lds, max_blocks)) return rc;
+    PP_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(Counters), c->stream));
+    PP_HIP(hipEventRecord(c->ev0, c->stream));
+    for (size_t i = 0; i < nph.size(); ++i) {
+        if (nph[i] == 0) continue;
+        kp.src = sources[i];
+        kp.src_ix = (uint32_t)i;
+        kp.n_photons = nph[i];
+        const int64_t need = (int64_t)((nph[i] + block - 1) / block);
+        const int blocks = (int)std::min<int64_t>(need, max_blocks);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(block), lds, c->stream, kp);
+        PP_HIP(hipGetLastError());
+        ++c->n_launch;
+    }
+    return PP_OK;
+}
+
+// The whole source list in one persistent launch (propagate_batch_kernel);
+// with use_graph, as a hipGraph of counter clear + propagation.
+int launch_batch(pp_ctx* c, const KParams& kp, const pp_source* sources,
+                 const std::vector<uint64_t>& nph, bool grid, bool hg, bool count, size_t lds,
+                 bool use_graph) {
+    using BatchFn = void (*)(KParams, BatchParams);
+    static const BatchFn kKernels[2][2][2] = {
+        {{propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, false>,
+          propagate_batch_kernel<PP_PHASE_HG, kSearchBrute, false>},
+         {propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, false>,
+          propagate_batch_kernel<PP_PHASE_HG, kSearchGrid, false>}},
+        {{propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, true>,
+          propagate_batch_kernel<PP_PHASE_HG, kSearchBrute, true>},
+         {propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, true>,
+          propagate_batch_kernel<PP_PHASE_HG, kSearchGrid, true>}}};
+    const BatchFn kern = kKernels[count][grid][hg];
+    const int block = c->cfg.block_size;
+    const size_t n = nph.size();
+    uint64_t total_ph = 0;
+    for (uint64_t v : nph) total_ph += v;
+    if (total_ph == 0) {
+        // nothing to run; the counters still have to read zero
+        PP_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(Counters), c->stream));
+        PP_HIP(hipEventRecord(c->ev0, c->stream));
+        return PP_OK;
+    }
+    int64_t max_blocks = 0;
+    if (int rc = max_resident_blocks(c, (const void*)kern, block, lds, max_blocks)) return rc;
+    const int64_t waves = max_blocks * (block / 64);
+    // chunk: about 64 chunks per resident wave, so that the last chunks
+    // taken leave little tail, and far more photons than one grab costs
+    uint64_t chunk = (uint64_t)c->cfg.chunk_photons;
+    if (chunk == 0) {
+        chunk = total_ph / ((uint64_t)waves * 64) / 64 * 64;
+        chunk = std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 8192);
+    }
+    // table: sources | photon counts | chunk prefix, each 256-byte aligned
+    const size_t off_n = ((n * sizeof(pp_source)) + 255) & ~(size_t)255;
+    const size_t off_p = off_n + ((n * 8 + 255) & ~(size_t)255);
+    const size_t bytes = off_p + (n + 1) * 8;
+    if (bytes > c->batch_cap) {
+        PP_HIP(hipStreamSynchronize(c->stream));
+        if (c->d_batch) PP_HIP(hipFree(c->d_batch));
+        if (c->h_batch) PP_HIP(hipHostFree(c->h_batch));
+        c->d_batch = c->h_batch = nullptr;
+        c->batch_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+        PP_HIP(hipMalloc(&c->d_batch, cap));
+        PP_HIP(hipHostMalloc(&c->h_batch, cap, hipHostMallocDefault));
+        c->batch_cap = cap;
+    }
+    char* h = (char*)c->h_batch;
+    std::memcpy(h, sources, n * sizeof(pp_source));
+    uint64_t* hn = (uint64_t*)(h + off_n);
+    uint64_t* hp = (uint64_t*)(h + off_p);
+    hp[0] = 0;
+    for (size_t i = 0; i < n; ++i) {
+        hn[i] = nph[i];
+        hp[i + 1] = hp[i] + (nph[i] + chunk - 1) / chunk;
+    }
+    const uint64_t n_chunks = hp[n];
+    char* d = (char*)c->d_batch;
+    BatchParams bp{};
+    bp.sources = (const pp_source*)d;
+    bp.n_photons = (const uint64_t*)(d + off_n);
+    bp.chunk_prefix = (const uint64_t*)(d + off_p);
+    bp.n_sources = (int32_t)n;
+    bp.chunk = (uint32_t)chunk;
+    const int64_t want = (int64_t)((n_chunks + block / 64 - 1) / (block / 64));
+    const unsigned blocks = (unsigned)std::min<int64_t>(want, max_blocks);
+    PP_HIP(hipMemcpyAsync(c->d_batch, c->h_batch, bytes, hipMemcpyHostToDevice, c->stream));
+    if (!use_graph) {
+        PP_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(Counters), c->stream));
+        PP_HIP(hipEventRecord(c->ev0, c->stream));
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(block), lds, c->stream, kp, bp);
+        PP_HIP(hipGetLastError());
+        c->n_launch = 1;
+        return PP_OK;
+    }
+    KParams kp_arg = kp;
+    void* args[] = {&kp_arg, &bp};
+    hipKernelNodeParams np{};
+    np.func = (void*)kern;
+    np.gridDim = dim3(blocks);
+    np.blockDim = dim3(block);
+    np.sharedMemBytes = (unsigned)lds;
+    np.kernelParams = args;
+    np.extra = nullptr;
+    const bool same_shape = c->graph_exec && c->graph_fn == (const void*)kern &&
+                            c->graph_blocks == blocks && c->graph_block == (unsigned)block &&
+                            c->graph_lds == lds;
+    bool updated = false;
+    if (same_shape)
+        updated = hipGraphExecKernelNodeSetParams(c->graph_exec, c->graph_prop, &np) == hipSuccess;
+    if (!updated) {
+        (void)hipGetLastError();
+        if (c->graph_exec) PP_HIP(hipGraphExecDestroy(c->graph_exec));
+        if (c->graph) PP_HIP(hipGraphDestroy(c->graph));
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        PP_HIP(hipGraphCreate(&c->graph, 0));
+        // the counters are cleared by a kernel node, not a memset node
+        Counters* ctr = c->d_counters;
+        void* cargs[] = {&ctr};
+        hipKernelNodeParams cp{};
+        cp.func = (void*)clear_counters_kernel;
+        cp.gridDim = dim3(1);
+        cp.blockDim = dim3(64);
+        cp.sharedMemBytes = 0;
+        cp.kernelParams = cargs;
+        cp.extra = nullptr;
+        hipGraphNode_t clear = nullptr;
+        PP_HIP(hipGraphAddKernelNode(&clear, c->graph, nullptr, 0, &cp));
+        PP_HIP(hipGraphAddKernelNode(&c->graph_prop, c->graph, &clear, 1, &np));
+        PP_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_fn = (const void*)kern;
+        c->graph_blocks = blocks;
+        c->graph_block = (unsigned)block;
+        c->graph_lds = lds;
+    }
+    PP_HIP(hipEventRecord(c->ev0, c->stream));
+    PP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    c->n_launch = 1;
     return PP_OK;
 }
 
@@ -567,6 +855,10 @@ int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* c
         return fail(PP_ERR_INVALID, "pp_create: block_size must be a multiple of 64 in [64,1024]");
     if (c.waves_per_simd < 0 || c.waves_per_simd > 32)
         return fail(PP_ERR_INVALID, "pp_create: waves_per_simd must be in [0,32]");
+    if (c.batch_mode < PP_BATCH_AUTO || c.batch_mode > PP_BATCH_GRAPH)
+        return fail(PP_ERR_INVALID, "pp_create: unknown batch_mode");
+    if (c.chunk_photons < 0 || c.chunk_photons % 64 || c.chunk_photons > (1 << 24))
+        return fail(PP_ERR_INVALID, "pp_create: chunk_photons must be a multiple of 64 in [0, 2^24]");
     auto* ctx = new pp_ctx();
     ctx->device = device;
     ctx->cfg = c;
@@ -619,8 +911,12 @@ int pp_destroy(pp_ctx* ctx) {
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_spec) (void)hipFree(ctx->d_spec);
     if (ctx->d_targets) (void)hipFree(ctx->d_targets);
-    if (ctx->d_cell_start) (void)hipFree(ctx->d_cell_start);
+    if (ctx->d_cells) (void)hipFree(ctx->d_cells);
     if (ctx->d_cell_items) (void)hipFree(ctx->d_cell_items);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    if (ctx->d_batch) (void)hipFree(ctx->d_batch);
+    if (ctx->h_batch) (void)hipHostFree(ctx->h_batch);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -762,29 +1058,7 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
         return fail(PP_ERR_INVALID, "pp_propagate: targets and tables exceed the LDS budget");
     const bool hg = ctx->medium.phase.kind == PP_PHASE_HG;
     const bool count = ctx->cfg.count_work != 0;
-    using KernelFn = void (*)(KParams);
-    // [count][grid][hg]
-    static const KernelFn kKernels[2][2][2] = {
-        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, false>,
-          propagate_kernel<PP_PHASE_HG, kSearchBrute, false>},
-         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, false>,
-          propagate_kernel<PP_PHASE_HG, kSearchGrid, false>}},
-        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, true>,
-          propagate_kernel<PP_PHASE_HG, kSearchBrute, true>},
-         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, true>,
-          propagate_kernel<PP_PHASE_HG, kSearchGrid, true>}}};
-    const KernelFn kern = kKernels[count][grid][hg];
     ctx->last_search = search;
-    int occ = 0;
-    PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds));
-    if (occ < 1) return fail(PP_ERR_INVALID, "pp_propagate: kernel does not fit one block per CU");
-    int64_t max_blocks = (int64_t)occ * ctx->n_cu;
-    if (ctx->cfg.waves_per_simd > 0)
-        max_blocks = std::max<int64_t>(1, (int64_t)ctx->n_cu * 4 * ctx->cfg.waves_per_simd * 64 / block);
-
-    PP_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(Counters), ctx->stream));
-    PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    ctx->n_launch = 0;
     KParams kp{};
     kp.med = ctx->medium;
     kp.k0 = (uint32_t)seed;
@@ -794,7 +1068,7 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     kp.n_targets = ctx->n_targets;
     if (grid) {
         kp.grid = ctx->grid;
-        kp.cell_start = ctx->d_cell_start;
+        kp.cells = ctx->d_cells;
         kp.cell_items = ctx->d_cell_items;
     }
     kp.spec_n = ctx->spec_n;
@@ -803,19 +1077,19 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     kp.spec = ctx->d_spec;
     kp.hits = ctx->hits;
     kp.counters = ctx->d_counters;
+    // photon count of src/photon_prop_cuda.jl:456 (Float32 product)
+    std::vector<uint64_t> nph((size_t)n_sources);
     for (int i = 0; i < n_sources; ++i) {
-        // photon count of src/photon_prop_cuda.jl:456 (Float32 product)
-        const float prod = (float)sources[i].photons * photon_scaling;
-        const int64_t nph = (int64_t)std::floor(prod);
-        if (nph <= 0) continue;
-        kp.src = sources[i];
-        kp.src_ix = (uint32_t)i;
-        kp.n_photons = (uint64_t)nph;
-        const int64_t need = (nph + block - 1) / block;
-        const int grid = (int)std::min<int64_t>(need, max_blocks);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, ctx->stream, kp);
-        PP_HIP(hipGetLastError());
-        ++ctx->n_launch;
+        const int64_t n = (int64_t)std::floor((float)sources[i].photons * photon_scaling);
+        nph[(size_t)i] = n > 0 ? (uint64_t)n : 0;
+    }
+    ctx->n_launch = 0;
+    const int mode = ctx->cfg.batch_mode == PP_BATCH_AUTO ? PP_BATCH_PERSISTENT : ctx->cfg.batch_mode;
+    if (mode == PP_BATCH_PER_SOURCE) {
+        if (int rc = launch_per_source(ctx, kp, sources, nph, grid, hg, count, lds)) return rc;
+    } else {
+        if (int rc = launch_batch(ctx, kp, sources, nph, grid, hg, count, lds, mode == PP_BATCH_GRAPH))
+            return rc;
     }
     PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     Counters hc{};

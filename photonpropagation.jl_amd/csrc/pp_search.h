@@ -7,11 +7,14 @@
 // BruteSearch tests every target (wave-uniform loop over LDS, broadcast
 // reads) - the reference's own algorithm.
 // GridSearch walks the step segment through a uniform grid of cubic cells
-// (3-D DDA, Amanatides & Woo) whose cells list the targets overlapping them,
-// so only targets near the segment are tested. Every target whose hit point
-// lies on the segment overlaps a visited cell (cells are binned with a
-// margin far above float rounding of the walk), and the smallest hit index
-// over all visited candidates is kept, so the result equals BruteSearch's.
+// (3-D DDA, Amanatides & Woo). Each cell record holds the bounding box of
+// the targets overlapping the cell; a slab test of the segment against that
+// box decides whether the cell's targets are tested. Every target whose hit
+// point lies on the segment overlaps a visited cell (cells are binned with a
+// margin far above float rounding of the walk) and lies inside that cell's
+// box (grown by a further margin, so the slab test can only err towards
+// testing), and the smallest hit index over all tested candidates is kept:
+// the result equals BruteSearch's.
 #pragma once
 
 #include "pp_device.h"
@@ -73,10 +76,20 @@ __host__ __device__ __forceinline__ int clampi(int v, int hi) {
     return v < 0 ? 0 : (v > hi ? hi : v);
 }
 
+__host__ __device__ __forceinline__ uint32_t float_bits(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __float_as_uint(x);
+#else
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    return u;
+#endif
+}
+
 struct GridSearch {
     const float4* tgt;
-    const uint32_t* cell_start;  // n_cells + 1
-    const uint16_t* items;       // target indices per cell
+    const float4* cells;    // 2 per cell: (lo xyz, start bits), (hi xyz, count bits)
+    const uint16_t* items;  // target indices per cell
     GridDesc g;
 
     __host__ __device__ __forceinline__ int operator()(f3 pos, f3 dir, float step, float& dt) const {
@@ -102,9 +115,9 @@ struct GridSearch {
         const float exc = dir_excess(dir);
         // entry cell
         const float px = pos.x + dir.x * t_in, py = pos.y + dir.y * t_in, pz = pos.z + dir.z * t_in;
-        int cx = clampi((int)floorf((px - g.ox) * g.inv_h), g.nx - 1);
-        int cy = clampi((int)floorf((py - g.oy) * g.inv_h), g.ny - 1);
-        int cz = clampi((int)floorf((pz - g.oz) * g.inv_h), g.nz - 1);
+        const int cx = clampi((int)floorf((px - g.ox) * g.inv_h), g.nx - 1);
+        const int cy = clampi((int)floorf((py - g.oy) * g.inv_h), g.ny - 1);
+        const int cz = clampi((int)floorf((pz - g.oz) * g.inv_h), g.nz - 1);
         const int sx = ix_ > 0.0f ? 1 : -1, sy = iy_ > 0.0f ? 1 : -1, sz = iz_ > 0.0f ? 1 : -1;
         // ray parameter of the next cell boundary on each axis
         float tmx = (g.ox + (float)(cx + (sx > 0)) * g.h - pos.x) * ix_;
@@ -118,28 +131,47 @@ struct GridSearch {
         int rz = sz > 0 ? g.nz - 1 - cz : cz;
         const int dcx = sx, dcy = sy * g.nx, dcz = sz * g.nx * g.ny;
         int c = (cz * g.ny + cy) * g.nx + cx;
+        float4 lo = cells[2 * c], hi = cells[2 * c + 1];
         const int max_visits = g.nx + g.ny + g.nz;
         for (int v = 0; v < max_visits; ++v) {
-            const uint32_t b = cell_start[c], e = cell_start[c + 1];
-            if (COUNT) {
-                w.cells += 1;
-                w.tests += e - b;
-            }
-            for (uint32_t k = b; k < e; ++k) {
-                const int t = items[k];
-                float d;
-                if (sphere_hit(tgt[t], pos, dir, exc, step, d) && (found < 0 || t < found)) {
-                    found = t;
-                    dt = d;
-                }
-            }
+            // next cell first, so that its record loads while this one is tested
             const float tn = fminf(fminf(tmx, tmy), tmz);
             const bool ax = tn == tmx;
             const bool ay = !ax && tn == tmy;
             const bool az = !ax && !ay;
             const int rem = ax ? rx : (ay ? ry : rz);
-            if (tn >= t_out || rem == 0) break;
-            c += ax ? dcx : (ay ? dcy : dcz);
+            const bool last = tn >= t_out || rem == 0;
+            const int cn = last ? c : c + (ax ? dcx : (ay ? dcy : dcz));
+            const float4 lo_n = cells[2 * cn], hi_n = cells[2 * cn + 1];
+            // this cell: slab test of the segment [0, step] against the box of
+            // its targets, then the targets themselves
+            const uint32_t cnt = float_bits(hi.w);
+            if (COUNT) w.cells += 1;
+            if (cnt != 0) {
+                const float ax0 = (lo.x - pos.x) * ix_, ax1 = (hi.x - pos.x) * ix_;
+                const float ay0 = (lo.y - pos.y) * iy_, ay1 = (hi.y - pos.y) * iy_;
+                const float az0 = (lo.z - pos.z) * iz_, az1 = (hi.z - pos.z) * iz_;
+                const float b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)),
+                                         fmaxf(fminf(az0, az1), 0.0f));
+                const float b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)),
+                                          fminf(fmaxf(az0, az1), step));
+                if (b_in <= b_out) {
+                    const uint32_t b = float_bits(lo.w);
+                    if (COUNT) w.tests += cnt;
+                    for (uint32_t k = b; k < b + cnt; ++k) {
+                        const int t = items[k];
+                        float d;
+                        if (sphere_hit(tgt[t], pos, dir, exc, step, d) && (found < 0 || t < found)) {
+                            found = t;
+                            dt = d;
+                        }
+                    }
+                }
+            }
+            if (last) break;
+            c = cn;
+            lo = lo_n;
+            hi = hi_n;
             rx -= ax;
             ry -= ay;
             rz -= az;

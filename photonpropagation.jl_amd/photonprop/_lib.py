@@ -46,6 +46,11 @@ SEARCH_AUTO = 0
 SEARCH_BRUTE = 1
 SEARCH_GRID = 2
 
+BATCH_AUTO = 0
+BATCH_PER_SOURCE = 1
+BATCH_PERSISTENT = 2
+BATCH_GRAPH = 3
+
 # enum pp_fn_id
 FN = dict(ISOTROPIC_DIR=0, ROT_FROM_EZ=1, HG_COS=2, ES_COS=3, UPDATE_DIR=4, SPHERE_ISEC=5,
           STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11)
@@ -79,7 +84,8 @@ class pp_target(Structure):
 class pp_config(Structure):
     _fields_ = [("block_size", c_int32), ("waves_per_simd", c_int32), ("search", c_int32),
                 ("grid_cell", c_float), ("deterministic_order", c_int32),
-                ("count_work", c_int32), ("reserved", c_int32 * 10)]
+                ("count_work", c_int32), ("batch_mode", c_int32),
+                ("chunk_photons", c_int32), ("reserved", c_int32 * 8)]
 
 
 assert ctypes.sizeof(pp_source) == 96

@@ -54,9 +54,9 @@ int main(int argc, char** argv) {
         return 3;
     }
     // exact-size copies so that the sanitizer sees every overrun
-    std::vector<uint32_t> start(t.start);
+    std::vector<float4> cells(t.cells);
     std::vector<uint16_t> items(t.items.begin(), t.items.begin() + std::max(t.g.n_items, 1));
-    GridSearch grid{tgt.data(), start.data(), items.data(), t.g};
+    GridSearch grid{tgt.data(), cells.data(), items.data(), t.g};
     BruteSearch brute{tgt.data(), n};
     long hits = 0, mism = 0;
     const float span = scene == 1 ? 30.0f : 300.0f;

@@ -259,3 +259,55 @@ def test_grid_search_1000_modules_matches_oracle(P, O, dev):
     n, d = compare_photon_by_photon(g, r, max_diff_frac=0.02)
     print(f"1000 modules, 6 sources: {n} common hits, {d} differing")
     assert n > 20
+
+
+def _run_cfg(P, wl, seed=None, **cfg_fields):
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 1
+    for k, v in cfg_fields.items():
+        setattr(cfg, k, v)
+    _, d = P.make_hit_buffers(capacity=1 << 21, config=cfg)
+    with d:
+        out = gpu_run(P, d, wl)
+        if seed is not None:  # a second call on the same context
+            wl.seed, old = seed, wl.seed
+            out = (out, gpu_run(P, d, wl))
+            wl.seed = old
+    return out
+
+
+def test_batch_modes_bit_identical(P, dev):
+    """One launch per source, the persistent work-stealing kernel (auto and
+    64-photon chunks) and its hipGraph give the same hits bit for bit,
+    including on a second call that re-parameterises the graph."""
+    wl = P.workloads.uneven_batch_1000(n_sources=40, seed=5, min_photons=1e3, max_photons=2e5)
+    wl.sources[3].photons = 0
+    wl.sources[17].photons = 37  # a partial wave
+    ref = _run_cfg(P, wl, batch_mode=P._lib.BATCH_PER_SOURCE)
+    ref2 = _run_cfg(P, wl, batch_mode=P._lib.BATCH_PER_SOURCE, seed=wl.seed + 1)[1]
+    assert len(ref[0]["wavelength"]) > 100
+    runs = {
+        "persistent": _run_cfg(P, wl, batch_mode=P._lib.BATCH_PERSISTENT),
+        "chunk64": _run_cfg(P, wl, batch_mode=P._lib.BATCH_PERSISTENT, chunk_photons=64,
+                            block_size=128),
+        "graph": _run_cfg(P, wl, batch_mode=P._lib.BATCH_GRAPH, seed=wl.seed + 1),
+    }
+    runs["graph"], graph2 = runs["graph"]
+    for name, (g, n) in runs.items():
+        assert n == ref[1], name
+        for k in g:
+            assert g[k].tobytes() == ref[0][k].tobytes(), (name, k)
+    assert graph2[1] == ref2[1]
+    for k in graph2[0]:
+        assert graph2[0][k].tobytes() == ref2[0][k].tobytes(), ("graph second call", k)
+
+
+def test_batch_mode_overflow_raises(P):
+    wl = P.workloads.uneven_batch_1000(n_sources=6, seed=8, min_photons=1e5, max_photons=2e5)
+    for mode in (P._lib.BATCH_PERSISTENT, P._lib.BATCH_GRAPH):
+        cfg = P._lib.pp_config()
+        cfg.batch_mode = mode
+        _, small = P.make_hit_buffers(capacity=16, config=cfg)
+        with small:
+            with pytest.raises(P.PhotonPropOOMException):
+                gpu_run(P, small, wl)
