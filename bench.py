@@ -62,7 +62,22 @@ def parse():
     ap.add_argument("--cpu-sample", type=float, default=0.0,
                     help="photons for the CPU baseline (0 = calibrate to ~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy twice (reference kernel of known "
+                         "size for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def calib_copy():
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes": a.numel() * 4, "copies": 2}), flush=True)
 
 
 def total_steps(hits, n_ph_sim, n_steps):
@@ -96,8 +111,24 @@ def cpu_baseline(wl_factory, photons):
                        f"(brute-force 320-sphere search, 15 steps, OpenMP {threads} threads)")
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the propagation kernel: calibrated FETCH_SIZE +
+    WRITE_SIZE from the committed rocprofv3 counter summary of this workload
+    (None if there is none)."""
+    path = os.path.join(ROOT, "profiles", f"propagate_batch_kernel_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def main():
     args = parse()
+    if args.calib_copy:
+        calib_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design only)"}))
@@ -180,6 +211,7 @@ def main():
     transc = (steps * (W_PHYS_TRANSC + (W_GRID_STEP_TRANSC if grid else 0.0))
               + tests * W_ISEC_TRANSC)
     achieved = flop / (kms * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(wl.name)
     ref_equiv = steps * (W_PHYS_FLOP + W_ISEC_FLOP * n_t) / (kms * 1e-3) / 1e12
     out = {
         "metric": "photons propagated/s on 1 MI355X",
@@ -210,7 +242,8 @@ def main():
         "cell_visits_per_step": cells / steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
                      "transcendental_per_s": transc / (kms * 1e-3),
                      "flop_per_launch": flop,
