@@ -199,7 +199,9 @@ int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes);
 int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* cfg);
 int pp_destroy(pp_ctx* ctx);
 
-/* Device memory of one hit record in bytes (capacity sizing). */
+/* Device memory of one hit record in bytes (capacity sizing): 64, one
+ * 64-byte aligned record per hit (photon ids below 2^40 per source, at most
+ * 65535 targets). */
 int64_t pp_hit_record_bytes(void);
 
 /* Medium used by the next pp_propagate (by-value kernel argument in the

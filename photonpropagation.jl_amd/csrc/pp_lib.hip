@@ -31,14 +31,25 @@ using pp::f3;
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kNumFloatCols = 12;  // pos xyz, dir xyz, init dir xyz, wl, time, dist
+// One hit record: 64 bytes, 64-byte aligned, so that a hit is written as two
+// full 32-byte sectors (four 16-byte stores by one lane).
+//   f[0..2] position, f[3..5] direction, f[6..8] initial direction,
+//   f[9] wavelength, f[10] time, f[11] distance travelled,
+//   id = photon id (bits 0..39) | module index, 1-based (bits 40..55),
+//   n_steps, source index.
+struct alignas(64) HitRecord {
+    float f[12];
+    uint64_t id;
+    int32_t n_steps;
+    int32_t source_ix;
+};
+static_assert(sizeof(HitRecord) == 64, "hit record is 64 bytes");
+constexpr uint64_t kPidBits = 40;
+constexpr uint64_t kMaxPhotonsPerSource = 1ull << kPidBits;
+constexpr int kMaxTargets = 65535;
 
 struct HitCols {
-    float* f[kNumFloatCols];
-    int32_t* module_ix;  // 1-based target index
-    int32_t* n_steps;
-    uint64_t* photon_id;
-    int32_t* source_ix;
+    HitRecord* rec;
     uint64_t capacity;
 };
 
@@ -199,22 +210,13 @@ __device__ __forceinline__ bool store_hits(bool has_hit, const HitRec& h, uint32
     if (!has_hit) return true;
     const uint64_t i = base + rank;
     if (i >= out.capacity) return false;
-    out.f[0][i] = h.pos.x;
-    out.f[1][i] = h.pos.y;
-    out.f[2][i] = h.pos.z;
-    out.f[3][i] = h.dir.x;
-    out.f[4][i] = h.dir.y;
-    out.f[5][i] = h.dir.z;
-    out.f[6][i] = h.init_dir.x;
-    out.f[7][i] = h.init_dir.y;
-    out.f[8][i] = h.init_dir.z;
-    out.f[9][i] = h.wl;
-    out.f[10][i] = h.time;
-    out.f[11][i] = h.dist;
-    out.module_ix[i] = h.module_ix;
-    out.n_steps[i] = h.n_steps;
-    out.photon_id[i] = h.pid;
-    out.source_ix[i] = (int32_t)src_ix;
+    float4* r = reinterpret_cast<float4*>(out.rec + i);
+    r[0] = make_float4(h.pos.x, h.pos.y, h.pos.z, h.dir.x);
+    r[1] = make_float4(h.dir.y, h.dir.z, h.init_dir.x, h.init_dir.y);
+    r[2] = make_float4(h.init_dir.z, h.wl, h.time, h.dist);
+    const uint64_t id = h.pid | ((uint64_t)(uint32_t)h.module_ix << kPidBits);
+    r[3] = make_float4(__uint_as_float((uint32_t)id), __uint_as_float((uint32_t)(id >> 32)),
+                       __int_as_float(h.n_steps), __uint_as_float(src_ix));
     return true;
 }
 
@@ -319,8 +321,16 @@ __device__ __forceinline__ void flush_counts(const KParams& p, unsigned long lon
 // 429-549) with photons grid-strided over the launch.
 // COUNT: instrumented variant that also counts photon steps, sphere tests
 // and grid-cell visits (roofline accounting); the results are identical.
+// Minimum waves per SIMD the register allocation must allow (the VGPR budget
+// is 512 / waves per lane).
+#ifndef PP_WAVES_PER_EU
+#define PP_WAVES_PER_EU 6
+#endif
+#define PP_PROPAGATE_ATTRS \
+    __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PP_WAVES_PER_EU)))
+
 template <int PHASE, int SEARCH, bool COUNT = false>
-__global__ void __launch_bounds__(1024) propagate_kernel(KParams p) {
+__global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
     const Lds l = stage_lds<SEARCH>(p);
     pp::Phase phase = p.med.phase;
     phase.kind = PHASE;
@@ -358,7 +368,7 @@ struct BatchParams {
 };
 
 template <int PHASE, int SEARCH, bool COUNT = false>
-__global__ void __launch_bounds__(1024) propagate_batch_kernel(KParams p, BatchParams b) {
+__global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams b) {
     const Lds l = stage_lds<SEARCH>(p);
     pp::Phase phase = p.med.phase;
     phase.kind = PHASE;
@@ -837,7 +847,7 @@ int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes) {
 }
 
 int64_t pp_hit_record_bytes(void) {
-    return kNumFloatCols * 4 + 4 + 4 + 8 + 4;
+    return (int64_t)sizeof(HitRecord);
 }
 
 int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* cfg) {
@@ -874,28 +884,13 @@ int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* c
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
         return cleanup(fail(PP_ERR_HIP, "stream/event creation failed"));
     const size_t cap = (size_t)hit_capacity;
-    // column layout: 12 float columns, module, n_steps, photon id, source;
-    // every column starts on a 256-byte boundary
-    const size_t col_bytes[kNumFloatCols + 4] = {
-        cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4,
-        cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 4, cap * 8, cap * 4};
-    size_t col_off[kNumFloatCols + 4];
-    size_t bytes = 0;
-    for (int i = 0; i < kNumFloatCols + 4; ++i) {
-        col_off[i] = bytes;
-        bytes += (col_bytes[i] + 255) & ~(size_t)255;
-    }
+    const size_t bytes = cap * sizeof(HitRecord);
     if (hipMalloc(&ctx->hit_mem, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return cleanup(fail(PP_ERR_NOMEM, "pp_create: cannot allocate " + std::to_string(bytes) +
                                               " bytes of hit buffer"));
     }
-    char* base = (char*)ctx->hit_mem;
-    for (int i = 0; i < kNumFloatCols; ++i) ctx->hits.f[i] = (float*)(base + col_off[i]);
-    ctx->hits.module_ix = (int32_t*)(base + col_off[kNumFloatCols]);
-    ctx->hits.n_steps = (int32_t*)(base + col_off[kNumFloatCols + 1]);
-    ctx->hits.photon_id = (uint64_t*)(base + col_off[kNumFloatCols + 2]);
-    ctx->hits.source_ix = (int32_t*)(base + col_off[kNumFloatCols + 3]);
+    ctx->hits.rec = (HitRecord*)ctx->hit_mem;
     ctx->hits.capacity = cap;
     if (hipMalloc(&ctx->d_counters, sizeof(Counters)) != hipSuccess)
         return cleanup(fail(PP_ERR_NOMEM, "pp_create: counters"));
@@ -1017,6 +1012,8 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
         return fail(PP_ERR_INVALID, "pp_propagate: bad source list");
     if (!ctx->has_medium || !ctx->has_spectrum || ctx->n_targets < 0)
         return fail(PP_ERR_STATE, "pp_propagate: set medium, spectrum and targets first");
+    if (ctx->n_targets > kMaxTargets)
+        return fail(PP_ERR_INVALID, "pp_propagate: more than 65535 targets");
     if (n_steps < 0 || n_steps > PP_MAX_STEPS)
         return fail(PP_ERR_INVALID, "pp_propagate: n_steps out of range");
     if (!(photon_scaling >= 0.0f) || !std::isfinite(photon_scaling))
@@ -1082,6 +1079,9 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     for (int i = 0; i < n_sources; ++i) {
         const int64_t n = (int64_t)std::floor((float)sources[i].photons * photon_scaling);
         nph[(size_t)i] = n > 0 ? (uint64_t)n : 0;
+        if (nph[(size_t)i] >= kMaxPhotonsPerSource)
+            return fail(PP_ERR_INVALID, "pp_propagate: source " + std::to_string(i) +
+                                            " has 2^40 or more photons after scaling");
     }
     ctx->n_launch = 0;
     const int mode = ctx->cfg.batch_mode == PP_BATCH_AUTO ? PP_BATCH_PERSISTENT : ctx->cfg.batch_mode;
@@ -1151,75 +1151,41 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
     if (n == 0) return PP_OK;
     if (int rc = set_dev(ctx)) return rc;
     const bool ordered = ctx->cfg.deterministic_order != 0;
-    // rows of the device buffer to read: [lo, lo+cnt)
-    int64_t lo = first, cnt = n;
-    if (ordered) {
-        if (!ctx->perm_valid) {
-            std::vector<uint64_t> pid((size_t)ctx->n_hits);
-            std::vector<int32_t> src((size_t)ctx->n_hits);
-            PP_HIP(hipMemcpy(pid.data(), ctx->hits.photon_id, pid.size() * 8, hipMemcpyDeviceToHost));
-            PP_HIP(hipMemcpy(src.data(), ctx->hits.source_ix, src.size() * 4, hipMemcpyDeviceToHost));
-            ctx->perm.resize((size_t)ctx->n_hits);
-            std::iota(ctx->perm.begin(), ctx->perm.end(), (int64_t)0);
-            std::sort(ctx->perm.begin(), ctx->perm.end(), [&](int64_t a, int64_t b) {
-                return src[a] != src[b] ? src[a] < src[b] : pid[a] < pid[b];
-            });
-            ctx->perm_valid = true;
-        }
-        lo = 0;
-        cnt = ctx->n_hits;
+    // records to read: [lo, lo+cnt) of the device buffer
+    const int64_t lo = ordered ? 0 : first, cnt = ordered ? ctx->n_hits : n;
+    std::vector<HitRecord> rec((size_t)cnt);
+    PP_HIP(hipMemcpy(rec.data(), ctx->hits.rec + lo, (size_t)cnt * sizeof(HitRecord),
+                     hipMemcpyDeviceToHost));
+    const uint64_t pid_mask = kMaxPhotonsPerSource - 1;
+    if (ordered && !ctx->perm_valid) {
+        ctx->perm.resize((size_t)ctx->n_hits);
+        std::iota(ctx->perm.begin(), ctx->perm.end(), (int64_t)0);
+        std::sort(ctx->perm.begin(), ctx->perm.end(), [&](int64_t a, int64_t b) {
+            const HitRecord &ra = rec[(size_t)a], &rb = rec[(size_t)b];
+            return ra.source_ix != rb.source_ix ? ra.source_ix < rb.source_ix
+                                                : (ra.id & pid_mask) < (rb.id & pid_mask);
+        });
+        ctx->perm_valid = true;
     }
-    auto row = [&](int64_t k) -> int64_t {
-        return ordered ? ctx->perm[(size_t)(first + k)] : (first + k - lo);
-    };
-    std::vector<float> fbuf((size_t)cnt);
-    auto get_f = [&](int col) -> int {
-        PP_HIP(hipMemcpy(fbuf.data(), ctx->hits.f[col] + lo, (size_t)cnt * 4, hipMemcpyDeviceToHost));
-        return PP_OK;
-    };
-    float* vec_out[3] = {pos3, dir3, init_dir3};
-    for (int v = 0; v < 3; ++v) {
-        if (!vec_out[v]) continue;
+    for (int64_t k = 0; k < n; ++k) {
+        const HitRecord& r = rec[(size_t)(ordered ? ctx->perm[(size_t)(first + k)] : k)];
         for (int c = 0; c < 3; ++c) {
-            if (int rc = get_f(3 * v + c)) return rc;
-            for (int64_t k = 0; k < n; ++k) vec_out[v][3 * k + c] = fbuf[(size_t)row(k)];
+            if (pos3) pos3[3 * k + c] = r.f[c];
+            if (dir3) dir3[3 * k + c] = r.f[3 + c];
+            if (init_dir3) init_dir3[3 * k + c] = r.f[6 + c];
         }
-    }
-    if (wavelength) {
-        if (int rc = get_f(9)) return rc;
-        for (int64_t k = 0; k < n; ++k) wavelength[k] = fbuf[(size_t)row(k)];
-    }
-    if (time) {
-        if (int rc = get_f(10)) return rc;
-        for (int64_t k = 0; k < n; ++k) {
+        if (wavelength) wavelength[k] = r.f[9];
+        if (time) {
             if (time_f64)
-                ((double*)time)[k] = fbuf[(size_t)row(k)];
+                ((double*)time)[k] = r.f[10];
             else
-                ((float*)time)[k] = fbuf[(size_t)row(k)];
+                ((float*)time)[k] = r.f[10];
         }
-    }
-    if (dist_travelled) {
-        if (int rc = get_f(11)) return rc;
-        for (int64_t k = 0; k < n; ++k) dist_travelled[k] = fbuf[(size_t)row(k)];
-    }
-    if (module_ix || n_steps || source_ix) {
-        std::vector<int32_t> ibuf((size_t)cnt);
-        int32_t* cols[3] = {ctx->hits.module_ix, ctx->hits.n_steps, ctx->hits.source_ix};
-        for (int c = 0; c < 3; ++c) {
-            if ((c == 0 && !module_ix) || (c == 1 && !n_steps) || (c == 2 && !source_ix)) continue;
-            PP_HIP(hipMemcpy(ibuf.data(), cols[c] + lo, (size_t)cnt * 4, hipMemcpyDeviceToHost));
-            for (int64_t k = 0; k < n; ++k) {
-                const int32_t v = ibuf[(size_t)row(k)];
-                if (c == 0) module_ix[k] = v;
-                else if (c == 1) n_steps[k] = v;
-                else source_ix[k] = v;
-            }
-        }
-    }
-    if (photon_id) {
-        std::vector<uint64_t> ubuf((size_t)cnt);
-        PP_HIP(hipMemcpy(ubuf.data(), ctx->hits.photon_id + lo, (size_t)cnt * 8, hipMemcpyDeviceToHost));
-        for (int64_t k = 0; k < n; ++k) photon_id[k] = ubuf[(size_t)row(k)];
+        if (dist_travelled) dist_travelled[k] = r.f[11];
+        if (module_ix) module_ix[k] = (int64_t)((r.id >> kPidBits) & 0xFFFF);
+        if (n_steps) n_steps[k] = r.n_steps;
+        if (photon_id) photon_id[k] = r.id & pid_mask;
+        if (source_ix) source_ix[k] = r.source_ix;
     }
     return PP_OK;
 }

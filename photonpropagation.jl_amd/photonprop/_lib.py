@@ -134,7 +134,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("PHOTONPROP_HIP_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise PhotonPropLibraryError(
             f"{p} not found: build it with __graft_entry__.build() "

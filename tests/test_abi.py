@@ -71,7 +71,8 @@ int main(void) {
 def test_record_bytes_and_version(P):
     lib = P._lib.load_library()
     assert lib.pp_abi_version() == 1
-    assert lib.pp_hit_record_bytes() == 68
+    # 64-byte records, the reference's sizeof(PhotonHit) (src/photon_prop_cuda.jl:872-875)
+    assert lib.pp_hit_record_bytes() == 64
 
 
 def test_create_without_device_fails_cleanly(P):
