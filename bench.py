@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--batch", choices=["auto", "per-source", "persistent", "graph"],
                     default="auto", help="source scheduling (pp_batch_mode)")
     ap.add_argument("--chunk", type=int, default=0, help="photons per work chunk (0 = auto)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuner (by default block size, chunk and grid cell are "
+                         "tuned on a 2e7-photon sample first, untimed)")
     ap.add_argument("--cpu-sample", type=float, default=0.0,
                     help="photons for the CPU baseline (0 = calibrate to ~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -158,6 +161,11 @@ def main():
     cfg.deterministic_order = 0
     _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
     P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
+    tuned = None
+    if args.autotune:
+        tuned = P.autotune.autotune(dev, wl.sources, wl.targets, wl.medium, wl.spectrum,
+                                    wl.seed, n_steps=wl.n_steps, sample_photons=2e7,
+                                    use_cache=False)
     pods = P.processing.sources_to_pod(wl.sources)
     n_hits = ctypes.c_int64()
     n_sim = ctypes.c_int64()
@@ -187,12 +195,7 @@ def main():
     grid = search_used.value == 2
     # work actually executed: one instrumented run (identical hits), untimed
     ccfg = P._lib.pp_config()
-    ccfg.search = cfg.search
-    ccfg.grid_cell = cfg.grid_cell
-    ccfg.block_size = cfg.block_size
-    ccfg.waves_per_simd = cfg.waves_per_simd
-    ccfg.batch_mode = cfg.batch_mode
-    ccfg.chunk_photons = cfg.chunk_photons
+    P._lib.check(lib.pp_get_config(dev.handle, ctypes.byref(ccfg)), "pp_get_config")
     ccfg.count_work = 1
     _, cdev = P.make_hit_buffers(capacity=1 << 24, config=ccfg)
     P.processing.stage(cdev, wl.targets, wl.medium, wl.spectrum)
@@ -235,6 +238,9 @@ def main():
                               "256 cascades/100 m tracks, log-uniform 1e4-1e7 photons, "
                               "10x10x10 spheres r=0.3 m")},
         "launches_per_step": int(dev.last_timing()[1]),
+        "launch": P.autotune.config_to_dict(ccfg),
+        "autotune": None if tuned is None else {"ms_on_sample": tuned["ms"],
+                                                "sample_photons": tuned["sample_photons"]},
         "photon_steps_per_s": steps / (ms_per_step / 1e3),
         "kernel_ms": kms,
         "hits_per_step": int(len(hits["n_steps"])),

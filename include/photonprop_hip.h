@@ -193,6 +193,10 @@ int pp_device_count(void);
  * reserved_memory_fraction, src/photon_prop_cuda.jl:879-880). */
 int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes);
 
+/* Architecture name (e.g. "gfx950", NUL-terminated, at most arch_len bytes)
+ * and compute-unit count of a device: the autotuner's cache key. */
+int pp_device_info(int device, char* arch, int arch_len, int32_t* n_cu);
+
 /* make_hit_buffers (src/photon_prop_cuda.jl:878-886): create a context on
  * `device` with a device hit buffer of `hit_capacity` records. The host
  * buffer stays with the caller (see pp_copy_hits). cfg may be NULL. */
@@ -249,6 +253,32 @@ void* pp_stream(pp_ctx* ctx);
  * The grid replaces the reference's linear scan (src/photon_prop_cuda.jl:
  * 497-506) with the same first-in-index result. */
 int pp_search_info(pp_ctx* ctx, int32_t* search, float* cell, int32_t* dims, int32_t* n_items);
+
+/* The context's configuration (pp_create's cfg after defaults) and its
+ * replacement; pp_set_config checks the same ranges as pp_create and takes
+ * effect at the next pp_propagate. */
+int pp_get_config(pp_ctx* ctx, pp_config* cfg);
+int pp_set_config(pp_ctx* ctx, const pp_config* cfg);
+
+/* Autotuner. The reference fixes its launch shape from the occupancy API
+ * (src/photon_prop_cuda.jl:914-920); this library searches it instead.
+ * Times pp_propagate on a sample of `sources` (photon counts scaled so that
+ * the sample holds about sample_photons photons; photon_scaling as in
+ * pp_propagate) by coordinate descent over
+ *   block size        64, 128, 256, 512, 1024
+ *   photons per lane  per work chunk: 1, 2, 4, 8, 16 (chunk_photons = 64 x),
+ *                     or the automatic chunk
+ *   grid cell edge    automatic, and 0.6x, 1.5x, 2.5x, 4x the automatic edge
+ *                     (grid search only)
+ * with the batch mode of the context (PP_BATCH_AUTO: persistent). Each
+ * candidate runs once untimed and twice timed (HIP events); the faster run
+ * counts. The fastest configuration is written to *best (if not NULL) and
+ * becomes the context's configuration; *best_ms is its kernel time on the
+ * sample. Hits of the tuning runs are discarded. Results never depend on
+ * the configuration, only the speed does. */
+int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t seed,
+                int32_t n_steps, float photon_scaling, int64_t sample_photons,
+                pp_config* best, float* best_ms);
 
 /* Photon steps, sphere tests and grid-cell visits executed by the last
  * pp_propagate of a context created with cfg->count_work (roofline

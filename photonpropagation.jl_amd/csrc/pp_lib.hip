@@ -650,6 +650,25 @@ int build_grid(pp_ctx* c, float cell_req) {
     return PP_OK;
 }
 
+// Defaults and range checks of a pp_config.
+int normalise_config(pp_config& c, const char* who) {
+    const std::string w(who);
+    if (c.block_size == 0) c.block_size = 256;
+    if (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64)
+        return fail(PP_ERR_INVALID, w + ": block_size must be a multiple of 64 in [64,1024]");
+    if (c.waves_per_simd < 0 || c.waves_per_simd > 32)
+        return fail(PP_ERR_INVALID, w + ": waves_per_simd must be in [0,32]");
+    if (c.search < PP_SEARCH_AUTO || c.search > PP_SEARCH_GRID)
+        return fail(PP_ERR_INVALID, w + ": unknown search");
+    if (!(c.grid_cell >= 0.0f) || !std::isfinite(c.grid_cell))
+        return fail(PP_ERR_INVALID, w + ": grid_cell must be finite and >= 0");
+    if (c.batch_mode < PP_BATCH_AUTO || c.batch_mode > PP_BATCH_GRAPH)
+        return fail(PP_ERR_INVALID, w + ": unknown batch_mode");
+    if (c.chunk_photons < 0 || c.chunk_photons % 64 || c.chunk_photons > (1 << 24))
+        return fail(PP_ERR_INVALID, w + ": chunk_photons must be a multiple of 64 in [0, 2^24]");
+    return PP_OK;
+}
+
 // Blocks that fill the device: occupancy limit or cfg.waves_per_simd.
 int max_resident_blocks(pp_ctx* c, const void* fn, int block, size_t lds, int64_t& out) {
     int occ = 0;
@@ -846,6 +865,20 @@ int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes) {
     return PP_OK;
 }
 
+int pp_device_info(int device, char* arch, int arch_len, int32_t* n_cu) {
+    int ndev = pp_device_count();
+    if (device < 0 || device >= ndev) return fail(PP_ERR_INVALID, "pp_device_info: no such device");
+    hipDeviceProp_t prop{};
+    PP_HIP(hipGetDeviceProperties(&prop, device));
+    if (arch && arch_len > 0) {
+        std::string a(prop.gcnArchName);
+        a = a.substr(0, a.find(':'));
+        std::snprintf(arch, (size_t)arch_len, "%s", a.c_str());
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    return PP_OK;
+}
+
 int64_t pp_hit_record_bytes(void) {
     return (int64_t)sizeof(HitRecord);
 }
@@ -860,15 +893,7 @@ int pp_create(pp_ctx** out, int device, int64_t hit_capacity, const pp_config* c
                                         " (" + std::to_string(ndev) + " visible)");
     pp_config c{};
     if (cfg) c = *cfg;
-    if (c.block_size == 0) c.block_size = 256;
-    if (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64)
-        return fail(PP_ERR_INVALID, "pp_create: block_size must be a multiple of 64 in [64,1024]");
-    if (c.waves_per_simd < 0 || c.waves_per_simd > 32)
-        return fail(PP_ERR_INVALID, "pp_create: waves_per_simd must be in [0,32]");
-    if (c.batch_mode < PP_BATCH_AUTO || c.batch_mode > PP_BATCH_GRAPH)
-        return fail(PP_ERR_INVALID, "pp_create: unknown batch_mode");
-    if (c.chunk_photons < 0 || c.chunk_photons % 64 || c.chunk_photons > (1 << 24))
-        return fail(PP_ERR_INVALID, "pp_create: chunk_photons must be a multiple of 64 in [0, 2^24]");
+    if (int rc = normalise_config(c, "pp_create")) return rc;
     auto* ctx = new pp_ctx();
     ctx->device = device;
     ctx->cfg = c;
@@ -1105,6 +1130,100 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     if (n_hits_out) *n_hits_out = ctx->n_hits;
     if (n_ph_sim_out) *n_ph_sim_out = ctx->n_ph_sim;
     if (hc.err) return fail(PP_ERR_HIT_OVERFLOW, "hit buffer overflow (PhotonPropOOMException)");
+    return PP_OK;
+}
+
+int pp_get_config(pp_ctx* ctx, pp_config* cfg) {
+    if (!ctx || !cfg) return fail(PP_ERR_INVALID, "pp_get_config: NULL argument");
+    *cfg = ctx->cfg;
+    return PP_OK;
+}
+
+int pp_set_config(pp_ctx* ctx, const pp_config* cfg) {
+    if (!ctx || !cfg) return fail(PP_ERR_INVALID, "pp_set_config: NULL argument");
+    pp_config c = *cfg;
+    if (int rc = normalise_config(c, "pp_set_config")) return rc;
+    ctx->cfg = c;
+    return PP_OK;
+}
+
+int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t seed,
+                int32_t n_steps, float photon_scaling, int64_t sample_photons, pp_config* best,
+                float* best_ms) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_autotune: NULL ctx");
+    if (n_sources <= 0 || !sources) return fail(PP_ERR_INVALID, "pp_autotune: no sources");
+    if (sample_photons <= 0) return fail(PP_ERR_INVALID, "pp_autotune: sample_photons must be > 0");
+    if (!(photon_scaling > 0.0f) || !std::isfinite(photon_scaling))
+        return fail(PP_ERR_INVALID, "pp_autotune: photon_scaling must be finite and > 0");
+    double total = 0.0;
+    for (int i = 0; i < n_sources; ++i) total += (double)sources[i].photons * photon_scaling;
+    if (!(total > 0.0)) return fail(PP_ERR_INVALID, "pp_autotune: the sources hold no photons");
+    const float scale = (float)std::min<double>(photon_scaling,
+                                                photon_scaling * (double)sample_photons / total);
+    const pp_config saved = ctx->cfg;
+    pp_config cur = saved;
+    if (cur.batch_mode == PP_BATCH_AUTO) cur.batch_mode = PP_BATCH_PERSISTENT;
+    cur.count_work = 0;
+    cur.waves_per_simd = 0;
+    auto run = [&](const pp_config& c, float& ms) -> int {
+        ctx->cfg = c;
+        float t = INFINITY;
+        for (int r = 0; r < 3; ++r) {
+            int64_t nh = 0, ns = 0;
+            const int rc = pp_propagate(ctx, sources, n_sources, seed, n_steps, scale, &nh, &ns);
+            if (rc != PP_OK) return rc;
+            if (r > 0) t = std::min(t, ctx->last_ms);
+        }
+        ms = t;
+        return PP_OK;
+    };
+    float t_best = 0.0f;
+    if (int rc = run(cur, t_best)) {
+        ctx->cfg = saved;
+        return rc;
+    }
+    // automatic grid edge of these targets, for the cell candidates
+    const bool grid = ctx->last_search == PP_SEARCH_GRID;
+    const float h_auto = grid ? ctx->grid.h : 0.0f;
+    std::vector<float> cells = {0.0f};
+    if (grid)
+        for (float f : {0.6f, 1.5f, 2.5f, 4.0f}) cells.push_back(f * h_auto);
+    const int blocks[] = {64, 128, 256, 512, 1024};
+    const int chunks[] = {0, 64, 128, 256, 512, 1024};
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int knob = 0; knob < 3; ++knob) {
+            const size_t n_cand = knob == 0 ? 5 : knob == 1 ? 6 : cells.size();
+            for (size_t k = 0; k < n_cand; ++k) {
+                pp_config c = cur;
+                if (knob == 0) c.block_size = blocks[k];
+                if (knob == 1) c.chunk_photons = chunks[k];
+                if (knob == 2) c.grid_cell = cells[k];
+                if (c.block_size == cur.block_size && c.chunk_photons == cur.chunk_photons &&
+                    c.grid_cell == cur.grid_cell)
+                    continue;
+                if (c.batch_mode == PP_BATCH_PER_SOURCE && knob == 1) continue;
+                float t = 0.0f;
+                const int rc = run(c, t);
+                if (rc == PP_ERR_HIT_OVERFLOW || rc == PP_ERR_INVALID) continue;  // e.g. LDS budget
+                if (rc != PP_OK) {
+                    ctx->cfg = saved;
+                    return rc;
+                }
+                if (t < t_best) {
+                    t_best = t;
+                    cur = c;
+                }
+            }
+        }
+    }
+    cur.count_work = saved.count_work;
+    cur.deterministic_order = saved.deterministic_order;
+    ctx->cfg = cur;
+    ctx->n_hits = 0;  // the tuning runs' hits are not results
+    ctx->n_ph_sim = 0;
+    ctx->perm_valid = false;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = t_best;
     return PP_OK;
 }
 

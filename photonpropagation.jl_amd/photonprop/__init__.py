@@ -6,7 +6,7 @@ hand-written HIP for gfx950, reached through its C ABI
 setup, emitter, target, medium and spectrum types and ``propagate_photons``
 with the reference's hit-table layout.
 """
-from . import _lib, workloads
+from . import _lib, autotune, workloads
 from .detection import (HomogeneousDetector, PhotonTarget, Spherical, make_detector_grid,
                         spherical_target, targets_to_pod)
 from .lightyield import (AxiconeEmitter, CherenkovEmitter, CherenkovTrackEmitter,

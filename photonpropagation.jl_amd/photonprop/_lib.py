@@ -115,6 +115,11 @@ SIGNATURES = [
     ("pp_search_info", c_int, [c_void_p, POINTER(c_int32), POINTER(c_float), POINTER(c_int32),
                                POINTER(c_int32)]),
     ("pp_work_counts", c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
+    ("pp_device_info", c_int, [c_int, ctypes.c_char_p, c_int, POINTER(c_int32)]),
+    ("pp_get_config", c_int, [c_void_p, POINTER(pp_config)]),
+    ("pp_set_config", c_int, [c_void_p, POINTER(pp_config)]),
+    ("pp_autotune", c_int, [c_void_p, c_void_p, c_int, c_uint64, c_int32, c_float, c_int64,
+                            POINTER(pp_config), POINTER(c_float)]),
     ("pp_eval_fn", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int64]),
 ]
 

@@ -311,3 +311,35 @@ def test_batch_mode_overflow_raises(P):
         with small:
             with pytest.raises(P.PhotonPropOOMException):
                 gpu_run(P, small, wl)
+
+
+def test_autotune_picks_a_candidate_and_keeps_results(P, tmp_path):
+    """The tuned launch is one of the searched candidates, is cached, and the
+    hits after tuning equal those of the default launch bit for bit."""
+    wl = P.workloads.uneven_batch_1000(n_sources=12, seed=4, min_photons=1e4, max_photons=3e5)
+    ref = _run_cfg(P, wl)
+    cache = str(tmp_path / "tune.json")
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 1
+    _, d = P.make_hit_buffers(capacity=1 << 21, config=cfg)
+    with d:
+        r = P.autotune.autotune(d, wl.sources, wl.targets, wl.medium, wl.spectrum, wl.seed,
+                                n_steps=wl.n_steps, sample_photons=2e6, cache_path=cache)
+        print("autotune:", r)
+        c = r["config"]
+        assert not r["cached"] and r["ms"] > 0
+        assert c["block_size"] in (64, 128, 256, 512, 1024)
+        assert c["chunk_photons"] in (0, 64, 128, 256, 512, 1024)
+        assert c["batch_mode"] == P._lib.BATCH_PERSISTENT
+        g = gpu_run(P, d, wl)
+    assert g[1] == ref[1]
+    for k in g[0]:
+        assert g[0][k].tobytes() == ref[0][k].tobytes(), k
+    _, d2 = P.make_hit_buffers(capacity=1 << 21, config=cfg)
+    with d2:
+        r2 = P.autotune.autotune(d2, wl.sources, wl.targets, wl.medium, wl.spectrum, wl.seed,
+                                 n_steps=wl.n_steps, sample_photons=2e6, cache_path=cache)
+        assert r2["cached"] and r2["config"] == c
+        got = P._lib.pp_config()
+        P._lib.load_library().pp_get_config(d2.handle, __import__("ctypes").byref(got))
+        assert P.autotune.config_to_dict(got) == c
