@@ -65,6 +65,22 @@ def lib():
         L.ppo_beta_sample.argtypes = [c_float, c_float, c_uint64, c_uint64, c_uint32]
         L.ppo_gamma_sample.restype = c_float
         L.ppo_gamma_sample.argtypes = [c_float, c_float, c_uint64, c_uint64, c_uint32]
+        L.ppo_bf16.restype = c_float
+        L.ppo_bf16.argtypes = [c_float]
+        L.ppo_digamma.restype = ctypes.c_double
+        L.ppo_digamma.argtypes = [ctypes.c_double]
+        L.ppo_loss_head.restype = None
+        L.ppo_loss_head.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]
+        L.ppo_mlp_forward.restype = c_int
+        L.ppo_mlp_forward.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
+        L.ppo_mlp_loss_grad.restype = c_int
+        L.ppo_mlp_loss_grad.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                        c_int, POINTER(ctypes.c_double), c_void_p]
+        L.ppo_adam.restype = None
+        L.ppo_adam.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int] + \
+            [ctypes.c_double] * 5
+        L.ppo_sur_features.restype = None
+        L.ppo_sur_features.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
         _lib = L
     return _lib
 
@@ -130,3 +146,66 @@ def propagate(sources, medium, targets, seed, n_steps=15, photon_scaling=1.0, mo
     if n < 0:
         raise OverflowError("oracle hit capacity exceeded")
     return hits[:n].copy(), nsim.value
+
+
+# ---- surrogate MLP (pp_mlp_oracle.c) ----
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def bf16(x):
+    """Round float32 values to bf16 (nearest even), returned as float32."""
+    x = np.ascontiguousarray(x, np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    special = (x.view(np.uint32) & 0x7F800000) == 0x7F800000
+    return np.where(special, x, r.view(np.float32))
+
+
+def mlp_forward(dims, params, x, quant=True):
+    dims = np.ascontiguousarray(dims, np.int32)
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    n = x.shape[0]
+    y = np.zeros((n, dims[-1]), np.float64)
+    assert lib().ppo_mlp_forward(_p(dims), len(dims) - 1, _p(params), _p(x), n, int(quant),
+                                 _p(y)) == 0
+    return y
+
+
+def mlp_loss_grad(dims, params, x, labels, quant=True):
+    dims = np.ascontiguousarray(dims, np.int32)
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    labels = np.ascontiguousarray(labels, np.float32)
+    g = np.zeros(params.size, np.float64)
+    loss = ctypes.c_double()
+    assert lib().ppo_mlp_loss_grad(_p(dims), len(dims) - 1, _p(params), _p(x), _p(labels),
+                                   x.shape[0], int(quant), ctypes.byref(loss), _p(g)) == 0
+    return loss.value, g
+
+
+def loss_head(y, labels):
+    y = np.ascontiguousarray(y, np.float64)
+    labels = np.ascontiguousarray(labels, np.float32)
+    n, n_out = y.shape
+    ls = np.zeros(n, np.float64)
+    dy = np.zeros((n, n_out), np.float64)
+    lib().ppo_loss_head(_p(y), n_out, _p(labels), n, _p(ls), _p(dy))
+    return ls, dy
+
+
+def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    p, g, m, v = (np.array(a, np.float32) for a in (p, g, m, v))
+    lib().ppo_adam(_p(p), _p(g), _p(m), _p(v), p.size, int(t), lr, b1, b2, eps, wd)
+    return p, m, v
+
+
+def sur_features(cascades, modules):
+    """cascades (n, 8): x, y, z, dx, dy, dz, energy, log_yield; modules (n, 3)."""
+    c = np.ascontiguousarray(cascades, np.float32)
+    m = np.ascontiguousarray(modules, np.float32)
+    out = np.zeros((c.shape[0], 8), np.float32)
+    lib().ppo_sur_features(_p(c), _p(m), c.shape[0], _p(out))
+    return out

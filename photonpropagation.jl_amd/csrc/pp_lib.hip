@@ -848,6 +848,13 @@ int pp_abi_version(void) { return PP_ABI_VERSION; }
 
 const char* pp_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+// used by pp_mlp.hip, the surrogate half of the library
+void pp_set_error_message(const char* msg) { g_err = msg; }
+
+extern "C" {
+
 int pp_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;

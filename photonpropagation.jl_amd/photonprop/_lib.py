@@ -93,6 +93,35 @@ assert ctypes.sizeof(pp_medium) == 96
 assert ctypes.sizeof(pp_target) == 32
 assert ctypes.sizeof(pp_config) == 64
 
+
+# ---- surrogate (include/photonprop_surrogate.h) ----
+SUR_N_FEATURES = 8
+SUR_N_OUT = 10
+SUR_BINS = 64
+SUR_TMAX = 2000.0
+SUR_TIME_SHIFT = 20.0
+SUR_LABEL_FLOATS = 2 + SUR_BINS
+
+
+class pp_cascade(Structure):
+    _fields_ = [("position", c_float * 3), ("direction", c_float * 3), ("energy", c_float),
+                ("log_yield", c_float)]
+
+
+class pp_sur_domain(Structure):
+    _fields_ = [("min_dist", c_float), ("max_dist", c_float), ("min_energy", c_float),
+                ("max_energy", c_float), ("module_radius", c_float), ("c_ref", c_float),
+                ("reserved", c_float * 2)]
+
+
+class pp_adam(Structure):
+    _fields_ = [("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
+                ("weight_decay", c_float), ("reserved", c_float * 3)]
+
+
+assert ctypes.sizeof(pp_cascade) == 32 and ctypes.sizeof(pp_sur_domain) == 32
+assert ctypes.sizeof(pp_adam) == 32
+
 # (name, restype, argtypes) of every exported entry point of include/photonprop_hip.h
 SIGNATURES = [
     ("pp_abi_version", c_int, []),
@@ -118,6 +147,23 @@ SIGNATURES = [
     ("pp_device_info", c_int, [c_int, ctypes.c_char_p, c_int, POINTER(c_int32)]),
     ("pp_get_config", c_int, [c_void_p, POINTER(pp_config)]),
     ("pp_set_config", c_int, [c_void_p, POINTER(pp_config)]),
+    ("pp_mlp_create", c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int,
+                              c_uint64]),
+    ("pp_mlp_destroy", c_int, [c_void_p]),
+    ("pp_mlp_param_count", c_int64, [c_void_p]),
+    ("pp_mlp_get_params", c_int, [c_void_p, c_void_p]),
+    ("pp_mlp_set_params", c_int, [c_void_p, c_void_p]),
+    ("pp_mlp_forward", c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    ("pp_mlp_loss_grad", c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(c_float),
+                                 c_void_p]),
+    ("pp_mlp_adam_step", c_int, [c_void_p, POINTER(pp_adam)]),
+    ("pp_mlp_train", c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int,
+                             POINTER(pp_adam), c_uint64, c_void_p]),
+    ("pp_mlp_eval_loss", c_int, [c_void_p, c_void_p, c_void_p, c_int64, POINTER(c_float)]),
+    ("pp_sur_features", c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    ("pp_sur_eval_pairs", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int,
+                                  POINTER(pp_sur_domain), c_void_p, c_void_p, c_void_p,
+                                  POINTER(c_float)]),
     ("pp_autotune", c_int, [c_void_p, c_void_p, c_int, c_uint64, c_int32, c_float, c_int64,
                             POINTER(pp_config), POINTER(c_float)]),
     ("pp_eval_fn", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int64]),

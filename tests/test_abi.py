@@ -12,12 +12,16 @@ import pytest
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "photonprop_hip.h")
+HEADERS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+                 if f.endswith(".h"))
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(pp_[a-z_0-9]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(pp_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_the_boundary():
@@ -45,8 +49,10 @@ def test_pod_layouts_match_header(P):
 #include <stddef.h>
 #include <stdio.h>
 #include "photonprop_hip.h"
+#include "photonprop_surrogate.h"
 int main(void) {
-  printf("%zu %zu %zu %zu\n", sizeof(pp_source), sizeof(pp_medium), sizeof(pp_target), sizeof(pp_config));
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(pp_source), sizeof(pp_medium), sizeof(pp_target),
+         sizeof(pp_config), sizeof(pp_cascade), sizeof(pp_sur_domain), sizeof(pp_adam));
   printf("%zu %zu %zu %zu\n", offsetof(pp_source, photons), offsetof(pp_source, long_a),
          offsetof(pp_medium, pressure), offsetof(pp_target, module_id));
   return 0;
@@ -63,7 +69,9 @@ int main(void) {
     offs = [int(x) for x in lines[1].split()]
     L = P._lib
     assert sizes == [ctypes.sizeof(L.pp_source), ctypes.sizeof(L.pp_medium),
-                     ctypes.sizeof(L.pp_target), ctypes.sizeof(L.pp_config)]
+                     ctypes.sizeof(L.pp_target), ctypes.sizeof(L.pp_config),
+                     ctypes.sizeof(L.pp_cascade), ctypes.sizeof(L.pp_sur_domain),
+                     ctypes.sizeof(L.pp_adam)]
     assert offs == [L.pp_source.photons.offset, L.pp_source.long_a.offset,
                     L.pp_medium.pressure.offset, L.pp_target.module_id.offset]
 
