@@ -12,16 +12,17 @@
  * state. Parameters are one flat fp32 vector, layer by layer: W_l (out x in,
  * row-major) then b_l.
  *
- * Outputs (n_out = 10): y0 = log of the expected hits per emitted photon;
- * y1..y3 mixture logits; y4..y6 log shapes; y7..y9 log scales - 4
- * (scale in ns). Time variable x = t - t_geo + PP_SUR_TIME_SHIFT, t_geo =
- * (r - module radius) / c_ref.
+ * Outputs (n_out = 10): y0 = log of the expected hits per unit exposure,
+ * exposure = photons x (R / 2r)^2 (the fraction of isotropic, unabsorbed
+ * light a sphere of radius R intercepts at distance r); y1..y3 mixture
+ * logits; y4..y6 log shapes; y7..y9 log scales - 4 (scale in ns). Time
+ * variable x = t - t_geo + PP_SUR_TIME_SHIFT, t_geo = (r - R) / c_ref.
  *
- * Training label of a sample: photons simulated n_mc, absorption-weighted hit
- * sum w, and the weighted histogram of x in PP_SUR_BINS bins over
- * [0, PP_SUR_TMAX) ns. Loss per sample:
+ * Training label of a sample: exposure n_mc (R/2r)^2 of the n_mc simulated
+ * photons, absorption-weighted hit sum w, and the weighted histogram of x in
+ * PP_SUR_BINS bins over [0, PP_SUR_TMAX) ns. Loss per sample:
  *   (lambda - w log lambda) + sum_bins h_j * (-log p(x_j)),
- *   lambda = n_mc exp(y0), p = the mixture density, x_j = bin centres;
+ *   lambda = exposure exp(y0), p = the mixture density, x_j = bin centres;
  * averaged over the batch.
  */
 #ifndef PHOTONPROP_SURROGATE_H
@@ -38,7 +39,7 @@ extern "C" {
 #define PP_SUR_BINS 64
 #define PP_SUR_TMAX 2000.0f
 #define PP_SUR_TIME_SHIFT 20.0f
-#define PP_SUR_LABEL_FLOATS (2 + PP_SUR_BINS) /* n_mc, w, hist[64] */
+#define PP_SUR_LABEL_FLOATS (2 + PP_SUR_BINS) /* exposure, w, hist[64] */
 
 typedef struct pp_mlp pp_mlp;
 
@@ -69,7 +70,7 @@ typedef struct pp_adam {
 
 /* Create a network on `device` with parameters drawn from `seed`
  * (U(-1/sqrt(fan_in), 1/sqrt(fan_in)); output layer scaled by 0.1 with
- * biases y0 = -12, log shapes log 2, log scales -1, 0.5, 2). max_batch bounds
+ * biases y0 = -3, log shapes log 2, log scales -1, 0.5, 2). max_batch bounds
  * the rows of one training step or inference chunk. */
 int pp_mlp_create(pp_mlp** out, int device, int n_in, int n_hidden, int n_layers, int n_out,
                   int max_batch, uint64_t seed);

@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(256) pair_features_kernel(PairArgs a) {
         if (a.xf) a.xf[i * a.n_in + k] = v;
         if (a.xb) a.xb[i * a.n_in + k] = (bf16)v;
     }
-    if (a.logy) a.logy[i] = c.log_yield;
+    // expected hits = yield x geometric exposure (R/2r)^2 x exp(y0)
+    if (a.logy) a.logy[i] = c.log_yield + 2.0f * logf(a.dom.module_radius / (2.0f * fmaxf(r, a.dom.module_radius)));
     if (a.fb)
         a.fb[i] = !(r >= a.dom.min_dist && r <= a.dom.max_dist && c.energy >= a.dom.min_energy &&
                     c.energy <= a.dom.max_energy);
@@ -694,7 +695,7 @@ int pp_mlp_create(pp_mlp** out, int device, int n_in, int n_hidden, int n_layers
             init[(size_t)(M->b_off[l] + i)] = (float)((2.0 * unif() - 1.0) * bound);
     }
     float* bl = init.data() + M->b_off[M->n_mat - 1];
-    const float head_bias[PP_SUR_N_OUT] = {-12.0f, 0.0f, 0.0f, 0.0f, 0.6931472f, 0.6931472f,
+    const float head_bias[PP_SUR_N_OUT] = {-3.0f, 0.0f, 0.0f, 0.0f, 0.6931472f, 0.6931472f,
                                            0.6931472f, -1.0f, 0.5f, 2.0f};
     for (int i = 0; i < PP_SUR_N_OUT; ++i) bl[i] = head_bias[i];
     if (int rc = upload_params(M, init)) return bail(rc);

@@ -184,7 +184,7 @@ class SurrogateSetup:
                                         PEMinus), self.medium, self.spectrum) for x in e]
         return np.log(np.asarray(ys, np.float64))
 
-    def domain(self, min_dist=5.0, max_dist=300.0, min_energy=100.0, max_energy=1e6):
+    def domain(self, min_dist=5.0, max_dist=800.0, min_energy=100.0, max_energy=1e6):
         d = _lib.pp_sur_domain()
         d.min_dist, d.max_dist = min_dist, max_dist
         d.min_energy, d.max_energy = min_energy, max_energy
@@ -202,7 +202,13 @@ def default_setup():
     return SurrogateSetup(medium, spec, dev)
 
 
-def sample_geometry(rng, n, r_range=(1.0, 300.0), e_range=(100.0, 1e6)):
+def exposure(n_photons, r, radius):
+    """Photons x (R / 2r)^2: the label normaliser of the expected hits."""
+    r = np.maximum(np.asarray(r, np.float64), radius)
+    return np.asarray(n_photons, np.float64) * (radius / (2.0 * r)) ** 2
+
+
+def sample_geometry(rng, n, r_range=(1.0, 800.0), e_range=(100.0, 1e6)):
     """Cascade vertices around a module at the origin: distance log-uniform,
     direction to the module uniform on the sphere, cascade axis isotropic,
     energy log-uniform."""
@@ -216,7 +222,7 @@ def sample_geometry(rng, n, r_range=(1.0, 300.0), e_range=(100.0, 1e6)):
     return pos.astype(np.float32), d.astype(np.float32), e
 
 
-def photons_for_distance(r, lo=2e4, hi=1e7, scale=2e4):
+def photons_for_distance(r, lo=2e4, hi=4e6, scale=2e4):
     """Photons simulated per sample: about constant expected hits up to r ~ 22 m."""
     return np.clip(scale * np.asarray(r) ** 2, lo, hi).astype(np.int64)
 
@@ -235,7 +241,7 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
     target = HomogeneousDetector(Spherical(np.float32([0, 0, 0]), setup.module_radius),
                                  np.pi * setup.module_radius ** 2, 1)
     labels = np.zeros((n_samples, LABEL), np.float32)
-    labels[:, 0] = nmc
+    labels[:, 0] = exposure(nmc, r, setup.module_radius)
     own = dev is None
     if own:
         _, dev = make_hit_buffers(capacity=1 << 25)
@@ -263,7 +269,9 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
             x = hits["time"].astype(np.float64) - tgeo + SHIFT
             binx = np.floor(x / (TMAX / BINS)).astype(np.int64)
             ok = (binx >= 0) & (binx < BINS)
-            np.add.at(labels, (s[ok], 2 + binx[ok]), w[ok].astype(np.float32))
+            flat = np.bincount((s[ok] - b0) * BINS + binx[ok], weights=w[ok],
+                               minlength=(b1 - b0) * BINS)
+            labels[b0:b1, 2:] += flat.reshape(b1 - b0, BINS).astype(np.float32)
             if log and (b0 // batch_sources) % 10 == 0:
                 log(f"  labels {b1}/{n_samples}: {photons:.3g} photons, "
                     f"{time.perf_counter() - t0:.1f} s")
@@ -279,3 +287,191 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
 
 def module_at_origin(n):
     return np.zeros((n, 3), np.float32)
+
+
+# ---------------------------------------------------------------------------
+# training, Monte Carlo judge, fallback
+# ---------------------------------------------------------------------------
+def train_surrogate(setup: SurrogateSetup, cascades, labels, *, steps=8000, batch=4096,
+                    lr=2e-3, lr_stages=4, seed=3, n_hidden=256, n_layers=4, val_frac=0.05,
+                    max_batch=1 << 16, log=print):
+    """Train a fresh network on (cascades, labels) from generate_training_data.
+    The learning rate halves over lr_stages equal stages. Returns (mlp, info)."""
+    n = cascades.shape[0]
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    n_val = max(1, int(val_frac * n))
+    val, tr = perm[:n_val], perm[n_val:]
+    mlp = MLP(n_hidden=n_hidden, n_layers=n_layers, max_batch=max_batch, seed=seed)
+    feats = mlp.features(cascades, module_at_origin(n))
+    t0 = time.perf_counter()
+    curves = []
+    per = max(1, steps // lr_stages)
+    for s in range(lr_stages):
+        curves.append(mlp.train(feats[tr], labels[tr], per, batch=batch, seed=seed * 7919 + s,
+                                lr=lr * 0.5 ** s))
+        if log:
+            log(f"  stage {s}: lr {lr * 0.5 ** s:.2g}, batch loss {curves[-1][-1]:.5g}, "
+                f"val loss {mlp.eval_loss(feats[val], labels[val]):.5g}")
+    secs = time.perf_counter() - t0
+    info = dict(train_seconds=secs, steps=per * lr_stages, batch=batch,
+                steps_per_s=per * lr_stages / secs, samples=int(len(tr)),
+                val_loss=mlp.eval_loss(feats[val], labels[val]),
+                loss_curve=np.concatenate(curves))
+    return mlp, info
+
+
+def array_events(rng, n, module_xyz, e_range=(1e3, 1e6), setup=None):
+    """Cascades at uniform positions inside the modules' bounding box, isotropic
+    axes, log-uniform energies: rows x, y, z, dx, dy, dz, energy, log_yield."""
+    lo, hi = module_xyz.min(axis=0), module_xyz.max(axis=0)
+    pos = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    e = np.exp(rng.uniform(np.log(e_range[0]), np.log(e_range[1]), n))
+    ly = (setup or default_setup()).log_yield(e)
+    return np.concatenate([pos, d, e[:, None], ly[:, None]], axis=1).astype(np.float32)
+
+
+def mc_expected_hits(setup: SurrogateSetup, events, targets, *, max_photons=1e9, seed=11,
+                     dev=None, events_per_call=16, with_times=False):
+    """The Monte Carlo's expected hits per (event, module): every event is an
+    ExtendedCherenkovEmitter with min(yield, max_photons) photons, each hit
+    weighted by its absorption weight and yield / photons. Returns dict with
+    'hits' (n_e, n_m), 'raw' hit counts, 'seconds', 'kernel_ms', 'photons' and,
+    with_times, per-hit (event, module, x = t - t_geo + 20 ns, weight)."""
+    ne, nm = events.shape[0], len(targets)
+    mxyz = np.array([t.shape.position for t in targets], np.float64)
+    own = dev is None
+    if own:
+        _, dev = make_hit_buffers(capacity=1 << 26)
+    exp_hits = np.zeros((ne, nm), np.float64)
+    w2 = np.zeros((ne, nm), np.float64)
+    raw = np.zeros((ne, nm), np.int64)
+    times = []
+    photons = 0
+    kernel_ms = 0.0
+    c_ref = setup.c_ref
+    t0 = time.perf_counter()
+    try:
+        for e0 in range(0, ne, events_per_call):
+            e1 = min(ne, e0 + events_per_call)
+            srcs, factor = [], []
+            for i in range(e0, e1):
+                y = float(np.exp(events[i, 7]))
+                n_ph = int(min(y, max_photons))
+                factor.append(y / n_ph)
+                srcs.append(ExtendedCherenkovEmitter(
+                    events[i, :3].copy(), events[i, 3:6].copy(), 0.0, n_ph,
+                    LongitudinalParameterisation.from_energy(float(events[i, 6]), setup.medium,
+                                                             PEMinus)))
+            hits, nsim = run_photon_prop_no_local_cache(
+                srcs, targets, setup.medium, setup.device_spectrum, seed + e0,
+                HostHitBuffer(np.float32), dev, n_steps=setup.n_steps, with_ids=True)
+            kernel_ms += dev.last_timing()[0]
+            photons += nsim
+            ev = hits["source_ix"].astype(np.int64) + e0
+            mod = hits["module_id"].astype(np.int64) - 1
+            w = np.exp(-hits["dist_travelled"].astype(np.float64) /
+                       setup.medium.absorption_length(hits["wavelength"].astype(np.float64)))
+            w *= np.asarray(factor)[ev - e0]
+            flat = ev * nm + mod
+            exp_hits += np.bincount(flat, weights=w, minlength=ne * nm).reshape(ne, nm)
+            w2 += np.bincount(flat, weights=w * w, minlength=ne * nm).reshape(ne, nm)
+            raw += np.bincount(flat, minlength=ne * nm).reshape(ne, nm)
+            if with_times:
+                r = np.linalg.norm(mxyz[mod] - events[ev, :3], axis=1)
+                x = hits["time"].astype(np.float64) - (r - setup.module_radius) / c_ref + SHIFT
+                times.append((ev, mod, x, w))
+    finally:
+        if own:
+            dev.close()
+    n_eff = np.where(w2 > 0, exp_hits ** 2 / np.maximum(w2, 1e-300), 0.0)
+    out = dict(hits=exp_hits, raw=raw, n_eff=n_eff, seconds=time.perf_counter() - t0,
+               kernel_ms=kernel_ms, photons=photons)
+    if with_times:
+        out["times"] = tuple(np.concatenate(c) for c in zip(*times)) if times else None
+    return out
+
+
+def gamma_mixture_logpdf(x, mix):
+    """log density of x (n,) under mixtures mix (n, 9): weights, shapes, scales."""
+    from scipy.special import gammaln
+    x = np.maximum(np.asarray(x, np.float64), 1e-6)[:, None]
+    w, k, th = mix[:, :3].astype(np.float64), mix[:, 3:6].astype(np.float64), \
+        mix[:, 6:9].astype(np.float64)
+    lp = np.log(np.maximum(w, 1e-300)) + (k - 1) * np.log(x) - x / th - k * np.log(th) - gammaln(k)
+    m = lp.max(axis=1, keepdims=True)
+    return (m + np.log(np.exp(lp - m).sum(axis=1, keepdims=True)))[:, 0]
+
+
+class SurrogateModel:
+    """Trained network + its domain; pairs outside the domain fall back to the
+    Monte Carlo."""
+
+    def __init__(self, mlp: MLP, setup: SurrogateSetup, domain=None):
+        self.mlp = mlp
+        self.setup = setup
+        self.domain = domain or setup.domain()
+
+    def expected_hits(self, events, targets, *, mc_fallback=True, max_photons=1e9, seed=5,
+                      dev=None, mixture=False):
+        """Expected hits for every (event, module) pair. Returns dict with
+        'hits', 'mixture', 'fallback' (mask), 'surrogate_ms' (device time of
+        the batched network evaluation) and 'mc_seconds' (fallback)."""
+        mxyz = np.array([t.shape.position for t in targets], np.float32)
+        hits, mix, fb, ms = self.mlp.eval_pairs(events, mxyz, self.domain, mixture=mixture)
+        hits = hits.astype(np.float64)
+        out = dict(hits=hits, mixture=mix, fallback=fb, surrogate_ms=ms, mc_seconds=0.0,
+                   mc_events=0)
+        if mc_fallback and fb.any():
+            rows = np.nonzero(fb.any(axis=1))[0]
+            mc = mc_expected_hits(self.setup, events[rows], targets, max_photons=max_photons,
+                                  seed=seed, dev=dev)
+            for k, e in enumerate(rows):
+                hits[e, fb[e]] = mc["hits"][k, fb[e]]
+            out["mc_seconds"] = mc["seconds"]
+            out["mc_events"] = int(len(rows))
+        return out
+
+
+def judge_events(model: SurrogateModel, events, targets, *, max_photons=1e9, min_eff_hits=2500,
+                 seed=21, dev=None):
+    """The full Monte Carlo judges the surrogate on held-out events: mean
+    relative error of the expected hits over the in-domain pairs whose MC
+    estimate has at least min_eff_hits effective hits (relative statistical
+    error <= 1/sqrt(min_eff_hits), 2% by default), and the mean per-hit
+    log-likelihood of the MC arrival-time residuals under the surrogate
+    mixture vs under the MC's own histogram."""
+    mc = mc_expected_hits(model.setup, events, targets, max_photons=max_photons, seed=seed,
+                          dev=dev, with_times=True)
+    sur = model.expected_hits(events, targets, mc_fallback=False, mixture=True)
+    sel = (~sur["fallback"]) & (mc["n_eff"] >= min_eff_hits)
+    rel = np.abs(sur["hits"][sel] / mc["hits"][sel] - 1.0)
+    stat = 1.0 / np.sqrt(mc["n_eff"][sel])
+    res = dict(pairs=int(sel.sum()), mean_rel_err=float(rel.mean()) if sel.any() else None,
+               median_rel_err=float(np.median(rel)) if sel.any() else None,
+               mean_signed_err=float(np.mean(sur["hits"][sel] / mc["hits"][sel] - 1.0))
+               if sel.any() else None,
+               mean_mc_stat_err=float(stat.mean()) if sel.any() else None,
+               mc_seconds=mc["seconds"], mc_kernel_ms=mc["kernel_ms"], mc_photons=mc["photons"],
+               surrogate_ms=sur["surrogate_ms"], n_events=int(events.shape[0]),
+               n_modules=len(targets))
+    if mc.get("times") is not None:
+        ev, mod, x, w = mc["times"]
+        keep = sel[ev, mod] & (x > 0) & (x < TMAX)
+        if keep.any():
+            ll_sur = gamma_mixture_logpdf(x[keep], sur["mixture"][ev[keep], mod[keep]])
+            # MC's own binned density per pair (same binning as the labels)
+            b = np.floor(x[keep] / (TMAX / BINS)).astype(np.int64)
+            pair = ev[keep] * len(targets) + mod[keep]
+            key = pair * BINS + b
+            uniq, inv = np.unique(key, return_inverse=True)
+            wb = np.bincount(inv, weights=w[keep])
+            up, pinv = np.unique(pair, return_inverse=True)
+            wp = np.bincount(pinv, weights=w[keep])
+            dens = wb[inv] / wp[pinv] / (TMAX / BINS)
+            ww = w[keep] / w[keep].sum()
+            res["time_ll_surrogate"] = float(np.sum(ww * ll_sur))
+            res["time_ll_mc_hist"] = float(np.sum(ww * np.log(dens)))
+    return res, mc, sur

@@ -185,11 +185,53 @@ def test_features_and_pair_evaluation(S, O):
         np.testing.assert_allclose(f, O.sur_features(ce, cm), atol=2e-5, rtol=1e-5)
         hits, mix, fb, ms = m.eval_pairs(ev, mods, dom, mixture=True)
         y = np.concatenate([m.forward(f[i:i + 1000]) for i in range(0, len(f), 1000)])
-        ref = np.exp(np.clip(y[:, 0], -60, 10) + ce[:, 7]).reshape(ne, nm)
-        np.testing.assert_allclose(hits, ref, rtol=2e-6)
-        r = np.linalg.norm(cm - ce[:, :3], axis=1).reshape(ne, nm)
+        r = np.linalg.norm(cm - ce[:, :3], axis=1)
+        expo = 2 * np.log(0.3 / (2 * np.maximum(r, 0.3)))  # (R / 2r)^2
+        ref = np.exp(np.clip(y[:, 0], -60, 10) + ce[:, 7] + expo).reshape(ne, nm)
+        np.testing.assert_allclose(hits, ref, rtol=2e-5)
+        r = r.reshape(ne, nm)
         e = ce[:, 6].reshape(ne, nm)
-        expect_fb = (r < 5) | (r > 300) | (e < 100) | (e > 1e6)
+        expect_fb = (r < 5) | (r > 800) | (e < 100) | (e > 1e6)
         assert np.array_equal(fb, expect_fb) and fb[0, 0]
         np.testing.assert_allclose(mix[..., :3].sum(-1), 1.0, rtol=1e-5)
         assert np.all(mix[..., 3:] > 0) and ms > 0
+
+
+def test_surrogate_gain_over_mc(S, P):
+    """End to end at test scale: Monte Carlo labels, training on the GPU, and
+    the full Monte Carlo as judge on held-out events in the 1000-module array.
+    The network's expected hits agree with the judge on well-measured pairs,
+    it evaluates pairs >= 100x faster than the Monte Carlo, and the Monte
+    Carlo alone, given the network's device time, is far less accurate."""
+    rng = np.random.default_rng(12)
+    st = S.default_setup()
+    _, dev = P.make_hit_buffers(capacity=1 << 25)
+    with dev:
+        c, lab, gs = S.generate_training_data(st, 40_000, seed=2, dev=dev, log=None)
+        mlp, info = S.train_surrogate(st, c, lab, steps=6000, lr_stages=4, log=None)
+        with mlp:
+            model = S.SurrogateModel(mlp, st)
+            targets = P.make_detector_grid(10, 10, 50.0, 10, 10.0, radius=0.3)
+            mxyz = np.array([t.shape.position for t in targets], np.float32)
+            ev = S.array_events(rng, 20, mxyz, setup=st)
+            res, mc, sur = S.judge_events(model, ev, targets, max_photons=1e9, dev=dev)
+            print("judge:", {k: v for k, v in res.items()})
+            assert res["pairs"] >= 50
+            assert res["mean_rel_err"] < 0.10
+            # throughput: 2e6 pairs through the batched network vs the Monte Carlo
+            big = S.array_events(rng, 2000, mxyz, setup=st)
+            _, _, fb, ms = mlp.eval_pairs(big, mxyz, model.domain)
+            sur_rate = big.shape[0] * len(targets) / (ms * 1e-3)
+            mc_rate = ev.shape[0] * len(targets) / (mc["kernel_ms"] * 1e-3)
+            print(f"pair evaluations/s: surrogate {sur_rate:.3g}, MC {mc_rate:.3g}")
+            assert sur_rate > 100 * mc_rate
+            # the Monte Carlo alone with the surrogate's device time for these events
+            t_sur = res["surrogate_ms"]
+            budget = mc["photons"] / mc["kernel_ms"] * t_sur / ev.shape[0]
+            cheap = S.mc_expected_hits(st, ev, targets, max_photons=max(budget, 64), seed=99,
+                                       dev=dev)
+            sel = (~sur["fallback"]) & (mc["n_eff"] >= 2500)
+            err_cheap = np.mean(np.abs(cheap["hits"][sel] / mc["hits"][sel] - 1))
+            print(f"equal-time MC ({budget:.3g} photons/event): mean rel err {err_cheap:.3f}; "
+                  f"surrogate {res['mean_rel_err']:.3f}")
+            assert err_cheap > 3 * res["mean_rel_err"]
