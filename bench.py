@@ -76,6 +76,10 @@ def parse():
     ap.add_argument("--cpu-sample", type=float, default=0.0,
                     help="photons for the CPU baseline (0 = calibrate to ~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--surrogate-kernels", action="store_true",
+                    help="only time the surrogate's kernels without hipGraph (for kernel "
+                         "tracing): 3 batched evaluations of 1e4 cascades x 1000 modules and "
+                         "50 eager training steps of batch 4096, then exit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy twice (reference kernel of known "
                          "size for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -123,6 +127,31 @@ def cpu_baseline(wl_factory, photons):
     return dict(value=runs[1], unit="photons/s", cores=threads, kind="port",
                 sample=f"median of 3 oracle runs of {photons:.3g} photons of the same event "
                        f"(brute-force 320-sphere search, 15 steps, OpenMP {threads} threads)")
+
+
+def surrogate_kernels():
+    import numpy as np
+
+    import photonprop as P
+    from photonprop import surrogate as S
+    st = S.default_setup()
+    targets = P.make_detector_grid(10, 10, 50.0, 10, 10.0, radius=0.3)
+    mxyz = np.array([t.shape.position for t in targets], np.float32)
+    events = S.array_events(np.random.default_rng(7), 10_000, mxyz, setup=st)
+    rng = np.random.default_rng(3)
+    x = rng.normal(0, 1, (4096, 8)).astype(np.float32)
+    lab = np.zeros((4096, 66), np.float32)
+    lab[:, 0] = 1e3
+    lab[:, 1] = rng.poisson(50, 4096)
+    lab[:, 2:] = rng.poisson(1.0, (4096, 64))
+    with S.MLP(seed=1) as mlp:
+        ev_ms = [mlp.eval_pairs(events, mxyz, st.domain())[3] for _ in range(3)]
+        t0 = time.perf_counter()
+        for _ in range(50):
+            mlp.loss_grad(x, lab)
+            mlp.adam_step(lr=1e-3)
+        train_s = (time.perf_counter() - t0) / 50
+    print(json.dumps({"eval_ms": ev_ms, "eager_train_step_ms": train_s * 1e3}), flush=True)
 
 
 def surrogate_leg(args):
@@ -198,6 +227,9 @@ def main():
     args = parse()
     if args.calib_copy:
         calib_copy()
+        return
+    if args.surrogate_kernels:
+        surrogate_kernels()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
