@@ -50,6 +50,10 @@ struct GemmArgs {
     const float* bias;
     const float* Z;  // pre-activation read by EPI_DSILU
     int ldz;
+    // split K: blockIdx.z covers K columns [z*k_chunk, (z+1)*k_chunk) and
+    // writes its partial product (EPI_F32 only) to C + z*c_stride
+    int k_chunk;
+    int64_t c_stride;
 };
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -97,7 +101,10 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     f32x16 acc = {};
-    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+    const int k_lo = blockIdx.z * g.k_chunk;
+    const int k_hi = k_lo + g.k_chunk < g.K ? k_lo + g.k_chunk : g.K;
+    float* const C = g.C ? g.C + blockIdx.z * g.c_stride : nullptr;
+    for (int k0 = k_lo; k0 < k_hi; k0 += kBK) {
         stage_tile<!A_COL>(g.A, g.lda, m0, k0, g.M, g.K, As);
         stage_tile<B_KMAJ>(g.B, g.ldb, n0, k0, g.N, g.K, Bs);
         __syncthreads();
@@ -120,12 +127,12 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         if (m >= g.M) continue;
         const float v = acc[r];
         if (EPI == EPI_F32) {
-            g.C[(size_t)m * g.ldc + n] = v;
+            C[(size_t)m * g.ldc + n] = v;
         } else if (EPI == EPI_BIAS_F32) {
-            g.C[(size_t)m * g.ldc + n] = v + bias;
+            C[(size_t)m * g.ldc + n] = v + bias;
         } else if (EPI == EPI_BIAS_SILU) {
             const float z = v + bias;
-            g.C[(size_t)m * g.ldc + n] = z;
+            if (C) C[(size_t)m * g.ldc + n] = z;
             g.H[(size_t)m * g.ldh + n] = (bf16)(z * sigmoid(z));
         } else {
             const float z = g.Z[(size_t)m * g.ldz + n];
@@ -135,19 +142,33 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     }
 }
 
-// out[c] = sum over rows of X[r*ld + c] (bias gradients), fp32 sum
+// Column sums of X (rows x cols, bf16) in row slices: blockIdx.y sums rows
+// [y*row_chunk, (y+1)*row_chunk) into part[y*cols + c] (bias gradients,
+// finished by reduce_partials_kernel in a fixed order).
 __global__ void __launch_bounds__(256) colsum_kernel(const bf16* X, int ld, int rows, int cols,
-                                                     float* out) {
-    __shared__ float part[4][64];
+                                                     int row_chunk, float* part) {
+    __shared__ float red[4][64];
     const int c = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+    const int lo = blockIdx.y * row_chunk;
+    const int hi = lo + row_chunk < rows ? lo + row_chunk : rows;
     float s = 0.0f;
     if (c < cols)
-        for (int r = r0; r < rows; r += 4) s += (float)X[(size_t)r * ld + c];
-    part[r0][threadIdx.x & 63] = s;
+        for (int r = lo + r0; r < hi; r += 4) s += (float)X[(size_t)r * ld + c];
+    red[r0][threadIdx.x & 63] = s;
     __syncthreads();
     if (r0 == 0 && c < cols)
-        out[c] = (part[0][threadIdx.x] + part[1][threadIdx.x]) +
-                 (part[2][threadIdx.x] + part[3][threadIdx.x]);
+        part[(size_t)blockIdx.y * cols + c] =
+            (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// out[i] = sum over s < S of part[s*n + i], in order s = 0..S-1
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const float* part, int S, int64_t n,
+                                                              float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < S; ++k) s += part[k * n + i];
+    out[i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -429,6 +450,8 @@ struct pp_mlp {
     bf16* dYb = nullptr;
     bf16 *dA = nullptr, *dB = nullptr;  // backward ping-pong
     float* Lb = nullptr;                // batch labels
+    float* part = nullptr;              // split-K / column-sum partials
+    int64_t part_cap = 0;               // floats
     float* xs = nullptr;                // fp32 staging of host features
     float* loss = nullptr;              // loss accumulator(s)
     int64_t loss_cap = 0;
@@ -452,14 +475,17 @@ namespace {
 template <bool A_COL, bool B_KMAJ, int EPI>
 int gemm(pp_mlp* M, const GemmArgs& a) {
     if (a.M <= 0 || a.N <= 0) return PP_OK;
-    const dim3 grid(blocks_for(a.N, kBN), blocks_for(a.M, kBM));
-    hipLaunchKernelGGL((gemm_kernel<A_COL, B_KMAJ, EPI>), grid, dim3(256), 0, M->stream, a);
+    GemmArgs b = a;
+    if (b.k_chunk <= 0) b.k_chunk = b.K;
+    const dim3 grid(blocks_for(b.N, kBN), blocks_for(b.M, kBM), blocks_for(b.K, b.k_chunk));
+    hipLaunchKernelGGL((gemm_kernel<A_COL, B_KMAJ, EPI>), grid, dim3(256), 0, M->stream, b);
     PP_HIP(hipGetLastError());
     return PP_OK;
 }
 
-// forward of `rows` rows whose bf16 input is in H[0]
-int forward(pp_mlp* M, int rows) {
+// forward of `rows` rows whose bf16 input is in H[0]; keep_z stores the fp32
+// pre-activations that the backward pass needs
+int forward(pp_mlp* M, int rows, bool keep_z) {
     for (int l = 0; l < M->n_mat; ++l) {
         GemmArgs a{};
         a.A = M->H[l];
@@ -471,7 +497,7 @@ int forward(pp_mlp* M, int rows) {
         a.K = M->dims[l];
         a.bias = M->p + M->b_off[l];
         if (l + 1 < M->n_mat) {
-            a.C = M->Z[l + 1];
+            a.C = keep_z ? M->Z[l + 1] : nullptr;
             a.ldc = a.N;
             a.H = M->H[l + 1];
             a.ldh = a.N;
@@ -501,6 +527,17 @@ int loss_head(pp_mlp* M, int rows, bool grad, bool per_step) {
     return PP_OK;
 }
 
+// K slices of a weight-gradient product of `outputs` elements over `rows`
+// batch rows: about 256 workgroups, at least 256 rows per slice, and within
+// the partial buffer.
+int split_count(const pp_mlp* M, int rows, int64_t outputs) {
+    const int64_t tiles = (outputs + (int64_t)kBM * kBN - 1) / ((int64_t)kBM * kBN);
+    int s = (int)std::max<int64_t>(1, 256 / std::max<int64_t>(tiles, 1));
+    s = std::min(s, std::max(1, rows / 256));
+    s = (int)std::min<int64_t>(s, std::max<int64_t>(1, M->part_cap / std::max<int64_t>(outputs, 1)));
+    return std::max(1, s);
+}
+
 // gradients of the parameters into M->g (internal layout) from dYb
 int backward(pp_mlp* M, int rows) {
     const bf16* dcur = M->dYb;
@@ -516,11 +553,28 @@ int backward(pp_mlp* M, int rows) {
         w.M = n_o;
         w.N = n_i;
         w.K = rows;
-        w.C = M->g + M->w_off[l];
         w.ldc = n_i;
+        // split the batch (K) so that the weight gradient fills the device;
+        // partial products are summed in a fixed order (deterministic)
+        const int s_w = split_count(M, rows, (int64_t)n_o * n_i);
+        w.k_chunk = (rows + s_w - 1) / s_w;
+        w.k_chunk = (w.k_chunk + kBK - 1) / kBK * kBK;
+        const int s_used = (rows + w.k_chunk - 1) / w.k_chunk;
+        w.C = s_used > 1 ? M->part : M->g + M->w_off[l];
+        w.c_stride = (int64_t)n_o * n_i;
         if (int rc = gemm<true, false, EPI_F32>(M, w)) return rc;
-        hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(n_o, 64)), dim3(256), 0, M->stream, dcur,
-                           n_o, rows, n_o, M->g + M->b_off[l]);
+        if (s_used > 1) {
+            hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(w.c_stride, 256)), dim3(256),
+                               0, M->stream, M->part, s_used, w.c_stride, M->g + M->w_off[l]);
+            PP_HIP(hipGetLastError());
+        }
+        const int rc_chunk = 64;
+        const int s_b = (rows + rc_chunk - 1) / rc_chunk;
+        hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(n_o, 64), s_b), dim3(256), 0, M->stream,
+                           dcur, n_o, rows, n_o, rc_chunk, M->part);
+        PP_HIP(hipGetLastError());
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(n_o, 256)), dim3(256), 0,
+                           M->stream, M->part, s_b, (int64_t)n_o, M->g + M->b_off[l]);
         PP_HIP(hipGetLastError());
         if (l == 0) break;
         GemmArgs d{};
@@ -681,6 +735,18 @@ int pp_mlp_create(pp_mlp** out, int device, int n_in, int n_hidden, int n_layers
         (void)hipGetLastError();
         return bail(fail(PP_ERR_NOMEM, "pp_mlp_create: device allocation failed"));
     }
+    {
+        int64_t wmax = 0, cmax = 0;
+        for (int l = 0; l < M->n_mat; ++l) {
+            wmax = std::max<int64_t>(wmax, (int64_t)M->dims[l + 1] * M->dims[l]);
+            cmax = std::max<int64_t>(cmax, M->dims[l + 1]);
+        }
+        M->part_cap = std::max<int64_t>(16 * wmax, ((int64_t)max_batch / 64 + 1) * cmax);
+        if (hipMalloc(&M->part, M->part_cap * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            return bail(fail(PP_ERR_NOMEM, "pp_mlp_create: device allocation failed"));
+        }
+    }
     if (int rc = ensure_loss(M, 1)) return bail(rc);
     // initial parameters
     std::mt19937_64 rng(seed);
@@ -708,6 +774,7 @@ int pp_mlp_destroy(pp_mlp* M) {
     (void)hipSetDevice(M->device);
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     void* ptrs[] = {M->p, M->g, M->m, M->v, M->pb, M->Y, M->dY, M->dYb, M->dA, M->dB, M->Lb, M->xs,
+                    M->part,
                     M->loss, M->step, M->dsX, M->dsL, M->logy, M->hits, M->mix, M->fb,
                     M->d_ev, M->d_mod};
     for (void* p : ptrs)
@@ -751,7 +818,7 @@ int pp_mlp_forward(pp_mlp* M, const float* x, int n, float* y) {
     if (n == 0) return PP_OK;
     if (int rc = set_dev(M)) return rc;
     if (int rc = stage_x(M, x, n)) return rc;
-    if (int rc = forward(M, n)) return rc;
+    if (int rc = forward(M, n, false)) return rc;
     PP_HIP(hipMemcpyAsync(y, M->Y, (size_t)n * M->dims.back() * 4, hipMemcpyDeviceToHost,
                           M->stream));
     PP_HIP(hipStreamSynchronize(M->stream));
@@ -767,7 +834,7 @@ int pp_mlp_loss_grad(pp_mlp* M, const float* x, const float* labels, int n, floa
     PP_HIP(hipMemcpyAsync(M->Lb, labels, (size_t)n * PP_SUR_LABEL_FLOATS * 4,
                           hipMemcpyHostToDevice, M->stream));
     PP_HIP(hipMemsetAsync(M->loss, 0, 4, M->stream));
-    if (int rc = forward(M, n)) return rc;
+    if (int rc = forward(M, n, true)) return rc;
     if (int rc = loss_head(M, n, true, false)) return rc;
     if (int rc = backward(M, n)) return rc;
     float l = 0.0f;
@@ -821,7 +888,7 @@ int pp_mlp_train(pp_mlp* M, const float* x, const float* labels, int64_t n, int 
     auto enqueue_step = [&]() -> int {
         hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(batch, 256)), dim3(256), 0, M->stream, ga);
         PP_HIP(hipGetLastError());
-        if (int rc = forward(M, batch)) return rc;
+        if (int rc = forward(M, batch, true)) return rc;
         if (int rc = loss_head(M, batch, true, true)) return rc;
         if (int rc = backward(M, batch)) return rc;
         return adam(M, *opt);
@@ -881,7 +948,7 @@ int pp_mlp_eval_loss(pp_mlp* M, const float* x, const float* labels, int64_t n, 
                               (size_t)rows * PP_SUR_LABEL_FLOATS * 4, hipMemcpyHostToDevice,
                               M->stream));
         PP_HIP(hipMemsetAsync(M->loss, 0, 4, M->stream));
-        if (int rc = forward(M, rows)) return rc;
+        if (int rc = forward(M, rows, false)) return rc;
         if (int rc = loss_head(M, rows, false, false)) return rc;
         float l = 0.0f;
         PP_HIP(hipMemcpyAsync(&l, M->loss, 4, hipMemcpyDeviceToHost, M->stream));
@@ -993,7 +1060,7 @@ int pp_sur_eval_pairs(pp_mlp* M, const pp_cascade* events, int n_events, const f
         hipLaunchKernelGGL(pair_features_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0,
                            M->stream, a);
         PP_HIP(hipGetLastError());
-        if (int rc = forward(M, rows)) return rc;
+        if (int rc = forward(M, rows, false)) return rc;
         hipLaunchKernelGGL(sur_head_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, M->stream,
                            M->Y, M->dims.back(), (int64_t)rows, M->logy, M->hits + r0,
                            mixture ? M->mix + r0 * 9 : nullptr);
