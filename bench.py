@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--search", choices=["auto", "brute", "grid"], default="auto")
     ap.add_argument("--cell", type=float, default=0.0, help="grid cell edge in m (0 = auto)")
+    ap.add_argument("--cell-z", type=float, default=0.0,
+                    help="grid cell edge along z in m (0 = cubic, < 0 = columns)")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = library default)")
     ap.add_argument("--wps", type=int, default=0, help="waves per SIMD for grid sizing (0 = auto)")
     ap.add_argument("--workload", choices=["cascade", "uneven"], default="cascade",
@@ -255,6 +257,7 @@ def main():
     cfg.chunk_photons = args.chunk
     cfg.search = {"auto": 0, "brute": 1, "grid": 2}[args.search]
     cfg.grid_cell = args.cell
+    cfg.grid_cell_z = args.cell_z
     cfg.block_size = args.block
     cfg.waves_per_simd = args.wps
     cfg.deterministic_order = 0

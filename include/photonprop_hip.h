@@ -152,7 +152,10 @@ typedef struct pp_config {
     int32_t batch_mode;         /* enum pp_batch_mode                          */
     int32_t chunk_photons;      /* photons per work chunk (multiple of 64);
                                    0 = auto                                    */
-    int32_t reserved[8];
+    float grid_cell_z;          /* grid cell edge along z (m): 0 = cubic cells,
+                                   < 0 = one layer over the targets' height
+                                   (columns, suited to vertical strings)       */
+    int32_t reserved[7];
 } pp_config; /* 64 bytes */
 
 /* ---- random stream layout (the reproducibility contract) ---------------
@@ -270,6 +273,7 @@ int pp_set_config(pp_ctx* ctx, const pp_config* cfg);
  *                     or the automatic chunk
  *   grid cell edge    automatic, and 0.6x, 1.5x, 2.5x, 4x the automatic edge
  *                     (grid search only)
+ *   grid cell shape   cubic, or columns over the whole height (grid only)
  * with the batch mode of the context (PP_BATCH_AUTO: persistent). Each
  * candidate runs once untimed and twice timed (HIP events); the faster run
  * counts. The fastest configuration is written to *best (if not NULL) and

@@ -36,16 +36,18 @@ inline float u32_as_float(uint32_t u) {
 }
 
 // Bin every target (centre xyz, radius) into the cells its bounding box plus
-// a margin m = 1e-3 h + 1 mm overlaps (the margin is far above the float
-// rounding of the DDA walk). Each cell lists its targets in ascending index
-// order and keeps the union of their boxes, grown by a further 1 cm + 1e-4 h,
-// so that a float slab test of a segment against it can only err towards
-// testing. cell_req <= 0 picks 1.5x the edge of one cell per target (about
-// 3.4 targets per cell; measured fastest on string detectors); the
-// edge grows by 1.25x until the tables fit `budget` bytes of LDS together
-// with the targets and n_spec spectrum knots. Returns false if nothing fits.
+// a margin m = 1e-3 max(h, hz) + 1 mm overlaps (the margin is far above the
+// float rounding of the DDA walk). Each cell lists its targets in ascending
+// index order and keeps the union of their boxes, grown by a further
+// 1 cm + 1e-4 max(h, hz), so that a float slab test of a segment against it can
+// only err towards testing. cell_req <= 0 picks 1.5x the edge of one cubic
+// cell per target (about 3.4 targets per cell); cell_z_req = 0 makes the cells
+// cubic, < 0 one layer over the whole height (columns: strings are vertical),
+// > 0 sets the z edge. The edges grow by 1.25x until the tables fit `budget`
+// bytes of LDS together with the targets and n_spec spectrum knots. Returns
+// false if nothing fits.
 inline bool build_grid_tables(const float* xyz, const float* radius, int n, float cell_req,
-                              int n_spec, size_t budget, GridTables& out) {
+                              float cell_z_req, int n_spec, size_t budget, GridTables& out) {
     if (n <= 0 || n > 65535) return false;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int i = 0; i < n; ++i)
@@ -58,20 +60,28 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
     float h = cell_req;
     if (!(h > 0.0f)) h = 1.5f * std::cbrt(ext[0] * ext[1] * ext[2] / (float)n);
     h = std::max(h, 1e-3f);
-    for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f) {
-        const float m = 1e-3f * h + 1e-3f;
+    float hz0 = cell_z_req > 0.0f ? std::max(cell_z_req, 1e-3f) : 0.0f;
+    for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f, hz0 *= 1.25f) {
+        // column cells: one layer whose edge covers the extent with margins
+        float hz = cell_z_req == 0.0f ? h : hz0;
+        float m = 1e-3f * std::max(h, hz) + 1e-3f;
+        if (cell_z_req < 0.0f) {
+            hz = ext[2] + 4.0f * (1e-3f * std::max(h, ext[2]) + 1e-3f) + 1e-2f;
+            m = 1e-3f * std::max(h, hz) + 1e-3f;
+        }
+        const float edge[3] = {h, h, hz};
         float o[3];
         int dim[3];
         for (int a = 0; a < 3; ++a) {
             o[a] = lo[a] - 2.0f * m;
-            dim[a] = std::max(1, (int)std::ceil((ext[a] + 4.0f * m) / h));
+            dim[a] = std::max(1, (int)std::ceil((ext[a] + 4.0f * m) / edge[a]));
         }
         const int64_t n_cells = (int64_t)dim[0] * dim[1] * dim[2];
         if (n_cells * 32 > (int64_t)budget) continue;
         auto range = [&](int i, int a, int& c0, int& c1) {
             const float p = xyz[3 * i + a], r = radius[i];
-            c0 = std::max(0, (int)std::floor((p - r - m - o[a]) / h));
-            c1 = std::min(dim[a] - 1, (int)std::floor((p + r + m - o[a]) / h));
+            c0 = std::max(0, (int)std::floor((p - r - m - o[a]) / edge[a]));
+            c1 = std::min(dim[a] - 1, (int)std::floor((p + r + m - o[a]) / edge[a]));
         };
         std::vector<uint32_t> count((size_t)n_cells, 0);
         int64_t n_items = 0;
@@ -94,7 +104,7 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         std::vector<uint32_t> fill(start.begin(), start.end() - 1);
         out.items.assign((size_t)std::max<int64_t>(n_items, 1), 0);
         std::vector<float> box_lo(3 * (size_t)n_cells, INFINITY), box_hi(3 * (size_t)n_cells, -INFINITY);
-        const float mb = m + 1e-2f + 1e-4f * h;
+        const float mb = m + 1e-2f + 1e-4f * std::max(h, hz);
         for (int i = 0; i < n; ++i) {
             int x0, x1, y0, y1, z0, z1;
             range(i, 0, x0, x1);
@@ -125,6 +135,8 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         g.oz = o[2];
         g.h = h;
         g.inv_h = 1.0f / h;
+        g.hz = hz;
+        g.inv_hz = 1.0f / hz;
         g.nx = dim[0];
         g.ny = dim[1];
         g.nz = dim[2];

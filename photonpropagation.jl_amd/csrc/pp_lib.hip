@@ -581,6 +581,7 @@ struct pp_ctx {
     // uniform grid over the targets, rebuilt when targets or cell size change
     pp::GridDesc grid{};
     float grid_cell_built = -1.0f;  // requested cell size of the built grid
+    float grid_cell_z_built = 0.0f;
     float4* d_cells = nullptr;
     uint16_t* d_cell_items = nullptr;
     size_t cell_cap = 0, item_cap = 0;
@@ -619,7 +620,7 @@ constexpr int kMaxLdsBytes = 160 * 1024;
 constexpr size_t kGridLdsBudget = 64 * 1024;  // grid tables + targets + spectrum per block
 
 // (Re)build the target grid for the requested cell edge and upload it.
-int build_grid(pp_ctx* c, float cell_req) {
+int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
     std::vector<float> xyz(3 * (size_t)c->n_targets);
     for (int i = 0; i < c->n_targets; ++i) {
         xyz[3 * i] = c->h_targets[i].x;
@@ -627,8 +628,8 @@ int build_grid(pp_ctx* c, float cell_req) {
         xyz[3 * i + 2] = c->h_targets[i].z;
     }
     pp::GridTables t;
-    if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, c->spec_n,
-                               kGridLdsBudget, t))
+    if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
+                               c->spec_n, kGridLdsBudget, t))
         return fail(PP_ERR_INVALID, "grid: no cell size fits the LDS budget");
     if (t.cells.size() > c->cell_cap) {
         if (c->d_cells) PP_HIP(hipFree(c->d_cells));
@@ -647,6 +648,7 @@ int build_grid(pp_ctx* c, float cell_req) {
     PP_HIP(hipMemcpy(c->d_cell_items, t.items.data(), t.items.size() * 2, hipMemcpyHostToDevice));
     c->grid = t.g;
     c->grid_cell_built = cell_req;
+    c->grid_cell_z_built = cell_z_req;
     return PP_OK;
 }
 
@@ -662,6 +664,8 @@ int normalise_config(pp_config& c, const char* who) {
         return fail(PP_ERR_INVALID, w + ": unknown search");
     if (!(c.grid_cell >= 0.0f) || !std::isfinite(c.grid_cell))
         return fail(PP_ERR_INVALID, w + ": grid_cell must be finite and >= 0");
+    if (!std::isfinite(c.grid_cell_z))
+        return fail(PP_ERR_INVALID, w + ": grid_cell_z must be finite");
     if (c.batch_mode < PP_BATCH_AUTO || c.batch_mode > PP_BATCH_GRAPH)
         return fail(PP_ERR_INVALID, w + ": unknown batch_mode");
     if (c.chunk_photons < 0 || c.chunk_photons % 64 || c.chunk_photons > (1 << 24))
@@ -1077,9 +1081,10 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     if (ctx->n_targets == 0) search = PP_SEARCH_BRUTE;
     const bool grid = search == PP_SEARCH_GRID;
     if (grid) {
-        const float want = ctx->cfg.grid_cell;
-        if (ctx->grid_cell_built != want || ctx->grid_cell_built < -0.5f)
-            if (int rc = build_grid(ctx, want)) return rc;
+        const float want = ctx->cfg.grid_cell, want_z = ctx->cfg.grid_cell_z;
+        if (ctx->grid_cell_built != want || ctx->grid_cell_z_built != want_z ||
+            ctx->grid_cell_built < -0.5f)
+            if (int rc = build_grid(ctx, want, want_z)) return rc;
     }
     const size_t lds = lds_bytes(ctx->n_targets, grid ? ctx->grid.n_cells : 0,
                                  grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
@@ -1197,16 +1202,19 @@ int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t s
         for (float f : {0.6f, 1.5f, 2.5f, 4.0f}) cells.push_back(f * h_auto);
     const int blocks[] = {64, 128, 256, 512, 1024};
     const int chunks[] = {0, 64, 128, 256, 512, 1024};
+    const float shapes[] = {0.0f, -1.0f};  // cubic cells, columns
     for (int pass = 0; pass < 2; ++pass) {
-        for (int knob = 0; knob < 3; ++knob) {
-            const size_t n_cand = knob == 0 ? 5 : knob == 1 ? 6 : cells.size();
+        for (int knob = 0; knob < 4; ++knob) {
+            if (knob >= 2 && !grid) continue;
+            const size_t n_cand = knob == 0 ? 5 : knob == 1 ? 6 : knob == 2 ? cells.size() : 2;
             for (size_t k = 0; k < n_cand; ++k) {
                 pp_config c = cur;
                 if (knob == 0) c.block_size = blocks[k];
                 if (knob == 1) c.chunk_photons = chunks[k];
                 if (knob == 2) c.grid_cell = cells[k];
+                if (knob == 3) c.grid_cell_z = shapes[k];
                 if (c.block_size == cur.block_size && c.chunk_photons == cur.chunk_photons &&
-                    c.grid_cell == cur.grid_cell)
+                    c.grid_cell == cur.grid_cell && c.grid_cell_z == cur.grid_cell_z)
                     continue;
                 if (c.batch_mode == PP_BATCH_PER_SOURCE && knob == 1) continue;
                 float t = 0.0f;

@@ -6,8 +6,8 @@
 //
 // BruteSearch tests every target (wave-uniform loop over LDS, broadcast
 // reads) - the reference's own algorithm.
-// GridSearch walks the step segment through a uniform grid of cubic cells
-// (3-D DDA, Amanatides & Woo). Each cell record holds the bounding box of
+// GridSearch walks the step segment through a uniform grid of box cells
+// (3-D DDA, Amanatides & Woo), cubic or taller along z. Each cell record holds the bounding box of
 // the targets overlapping the cell; a slab test of the segment against that
 // box decides whether the cell's targets are tested. Every target whose hit
 // point lies on the segment overlaps a visited cell (cells are binned with a
@@ -23,7 +23,8 @@ namespace pp {
 
 struct GridDesc {
     float ox, oy, oz;  // lower corner
-    float h, inv_h;    // cell edge
+    float h, inv_h;    // cell edge along x and y
+    float hz, inv_hz;  // cell edge along z (strings are vertical: cells may be columns)
     int nx, ny, nz;
     int n_cells, n_items;
 };
@@ -104,7 +105,7 @@ struct GridSearch {
         const float ix_ = fast_rcp(nonzero(dir.x)), iy_ = fast_rcp(nonzero(dir.y)),
                     iz_ = fast_rcp(nonzero(dir.z));
         // clip the segment [0, step] to the grid box
-        const float ex = g.ox + g.nx * g.h, ey = g.oy + g.ny * g.h, ez = g.oz + g.nz * g.h;
+        const float ex = g.ox + g.nx * g.h, ey = g.oy + g.ny * g.h, ez = g.oz + g.nz * g.hz;
         const float tx0 = (g.ox - pos.x) * ix_, tx1 = (ex - pos.x) * ix_;
         const float ty0 = (g.oy - pos.y) * iy_, ty1 = (ey - pos.y) * iy_;
         const float tz0 = (g.oz - pos.z) * iz_, tz1 = (ez - pos.z) * iz_;
@@ -117,13 +118,13 @@ struct GridSearch {
         const float px = pos.x + dir.x * t_in, py = pos.y + dir.y * t_in, pz = pos.z + dir.z * t_in;
         const int cx = clampi((int)floorf((px - g.ox) * g.inv_h), g.nx - 1);
         const int cy = clampi((int)floorf((py - g.oy) * g.inv_h), g.ny - 1);
-        const int cz = clampi((int)floorf((pz - g.oz) * g.inv_h), g.nz - 1);
+        const int cz = clampi((int)floorf((pz - g.oz) * g.inv_hz), g.nz - 1);
         const int sx = ix_ > 0.0f ? 1 : -1, sy = iy_ > 0.0f ? 1 : -1, sz = iz_ > 0.0f ? 1 : -1;
         // ray parameter of the next cell boundary on each axis
         float tmx = (g.ox + (float)(cx + (sx > 0)) * g.h - pos.x) * ix_;
         float tmy = (g.oy + (float)(cy + (sy > 0)) * g.h - pos.y) * iy_;
-        float tmz = (g.oz + (float)(cz + (sz > 0)) * g.h - pos.z) * iz_;
-        const float tdx = g.h * fabsf(ix_), tdy = g.h * fabsf(iy_), tdz = g.h * fabsf(iz_);
+        float tmz = (g.oz + (float)(cz + (sz > 0)) * g.hz - pos.z) * iz_;
+        const float tdx = g.h * fabsf(ix_), tdy = g.h * fabsf(iy_), tdz = g.hz * fabsf(iz_);
         // cells left before leaving the grid along each axis, cell index and
         // its increments: the walk advances with selects, not branches
         int rx = sx > 0 ? g.nx - 1 - cx : cx;

@@ -85,7 +85,8 @@ class pp_config(Structure):
     _fields_ = [("block_size", c_int32), ("waves_per_simd", c_int32), ("search", c_int32),
                 ("grid_cell", c_float), ("deterministic_order", c_int32),
                 ("count_work", c_int32), ("batch_mode", c_int32),
-                ("chunk_photons", c_int32), ("reserved", c_int32 * 8)]
+                ("chunk_photons", c_int32), ("grid_cell_z", c_float),
+                ("reserved", c_int32 * 7)]
 
 
 assert ctypes.sizeof(pp_source) == 96

@@ -20,8 +20,8 @@ from . import _lib
 from .detection import targets_to_pod
 from .processing import DeviceHitBuffer, sources_to_pod, stage
 
-CONFIG_FIELDS = ("block_size", "waves_per_simd", "search", "grid_cell", "batch_mode",
-                 "chunk_photons")
+CONFIG_FIELDS = ("block_size", "waves_per_simd", "search", "grid_cell", "grid_cell_z",
+                 "batch_mode", "chunk_photons")
 
 
 def default_cache_path() -> str:

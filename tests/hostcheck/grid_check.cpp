@@ -3,7 +3,8 @@
 // the propagation kernel inlines) on the CPU over random segments, counting
 // disagreements. Built by tests/test_grid_host.py with hipcc for the host,
 // under AddressSanitizer, so any out-of-bounds table access aborts the run.
-//   usage: grid_check <scene> <seed> <n_rays> <cell>   (cell <= 0: auto)
+//   usage: grid_check <scene> <seed> <n_rays> <cell> [cell_z]
+//          (cell <= 0: auto; cell_z 0: cubic, < 0: columns, > 0: z edge)
 //   prints: rays hits mismatches nx ny nz n_items
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +21,7 @@ int main(int argc, char** argv) {
     const unsigned seed = (unsigned)atoi(argv[2]);
     const long n_rays = atol(argv[3]);
     const float cell = (float)atof(argv[4]);
+    const float cell_z = argc > 5 ? (float)atof(argv[5]) : 0.0f;
     std::mt19937_64 rng(seed);
     std::uniform_real_distribution<float> U(0.0f, 1.0f);
     std::normal_distribution<float> N(0.0f, 1.0f);
@@ -49,7 +51,7 @@ int main(int argc, char** argv) {
     for (int i = 0; i < n; ++i)
         tgt[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], rad[i] * rad[i]);
     GridTables t;
-    if (!build_grid_tables(xyz.data(), rad.data(), n, cell, 600, 64 * 1024, t)) {
+    if (!build_grid_tables(xyz.data(), rad.data(), n, cell, cell_z, 600, 64 * 1024, t)) {
         printf("build failed\n");
         return 3;
     }

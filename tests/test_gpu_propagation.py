@@ -217,8 +217,9 @@ def test_statistical_parity_cascade(P, O, dev):
     del mods
 
 
-@pytest.mark.parametrize("cell", [0.0, 4.0, 30.0])
-def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell):
+@pytest.mark.parametrize("cell,cell_z", [(0.0, 0.0), (4.0, 0.0), (30.0, 0.0), (0.0, -1.0),
+                                         (20.0, 60.0)])
+def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell, cell_z):
     """The uniform-grid search returns the reference's first-in-index target:
     hits identical bit for bit to the brute-force scan."""
     import ctypes
@@ -228,6 +229,7 @@ def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell):
         cfg = P._lib.pp_config()
         cfg.search = search
         cfg.grid_cell = cell
+        cfg.grid_cell_z = cell_z
         cfg.deterministic_order = 1
         _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
         with d:

@@ -24,13 +24,20 @@ def grid_check():
     return exe
 
 
-@pytest.mark.parametrize("scene,cell", [(0, 0.0), (0, 3.0), (0, 25.0), (1, 0.0), (1, 0.7),
-                                        (2, 0.0), (2, 12.0)])
-def test_grid_equals_brute_force(grid_check, scene, cell):
-    r = subprocess.run([grid_check, str(scene), "17", "200000", str(cell)], capture_output=True,
-                       text=True, env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+@pytest.mark.parametrize("scene,cell,cell_z", [
+    (0, 0.0, 0.0), (0, 3.0, 0.0), (0, 25.0, 0.0), (1, 0.0, 0.0), (1, 0.7, 0.0),
+    (2, 0.0, 0.0), (2, 12.0, 0.0),
+    # column cells (one layer over the height) and explicit tall cells
+    (0, 0.0, -1.0), (0, 50.0, -1.0), (1, 2.0, -1.0), (2, 0.0, -1.0), (2, 30.0, 75.0)])
+def test_grid_equals_brute_force(grid_check, scene, cell, cell_z):
+    r = subprocess.run([grid_check, str(scene), "17", "200000", str(cell), str(cell_z)],
+                       capture_output=True, text=True,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
     rays, hits, mism, nx, ny, nz, n_items = map(int, r.stdout.split()[-7:])
-    print(f"scene {scene} cell {cell}: grid {nx}x{ny}x{nz}, {n_items} entries, {hits} hits")
+    print(f"scene {scene} cell {cell}/{cell_z}: grid {nx}x{ny}x{nz}, {n_items} entries, "
+          f"{hits} hits")
     assert hits > 1000
     assert mism == 0
+    if cell_z < 0:
+        assert nz == 1
