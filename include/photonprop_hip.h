@@ -178,6 +178,7 @@ typedef struct pp_config {
 #define PP_SUB_BETA_X 1u
 #define PP_SUB_BETA_Y 2u
 #define PP_SUB_NORMAL 3u
+#define PP_CTR_PMT (PP_CTR_INIT_EXTRA | 2u) /* PMT selection: [u_hit, u_pmt, -, -] */
 #define PP_GAMMA_MAX_BLOCKS 64u
 #define PP_MAX_STEPS 0x3FFFFFFF
 
@@ -262,6 +263,40 @@ int pp_search_info(pp_ctx* ctx, int32_t* search, float* cell, int32_t* dims, int
  * effect at the next pp_propagate. */
 int pp_get_config(pp_ctx* ctx, pp_config* cfg);
 int pp_set_config(pp_ctx* ctx, const pp_config* cfg);
+
+/* ---- PMT-level detection of the POM (src/pom.jl:108-305) -----------------
+ * check_pmt_hit(::POM{<:POMAcceptance}) for a batch of photon hits on the
+ * host: every hit on a module's sphere is accepted by one of its 16 PMTs or
+ * by none. With the hit position relative to the module centre in units of
+ * its radius (rel_pos), for PMT j of group g(j) (1 near the equator, 2 at
+ * +-57.5 deg), pt_j = acos(rel_pos . pmt_j):
+ *   p_j = Rayleigh_pdf(pt_j; sigma_g) / (0.5 sin pt_j) * acc_g(wavelength) / 8
+ * (0 where sin pt_j = 0; acc_g linear in the table, 0 outside). The hit is
+ * detected if u_hit < sum_j p_j, by PMT j drawn with weights p_j from u_pmt.
+ * The draws come from the photon's Philox stream (word PP_CTR_PMT) under
+ * key `seed`, so a photon's fate does not depend on the batch. PMT
+ * directions: orientation x R x (sin t cos f, sin t sin f, cos t) for the
+ * reference's (t, f) of each PMT, R the rotation taking e_z to e_x. */
+typedef struct pp_pom_model {
+    float sigma_grp1, sigma_grp2;  /* Rayleigh widths (rad), groups 1 and 2 */
+    float orientation[9];          /* module orientation, row-major         */
+    int32_t n_wl;                  /* acceptance table length (>= 2)        */
+    float reserved[4];
+} pp_pom_model; /* 64 bytes */
+
+/* PMT directions (16 x 3, row-major) of the model: the reference's
+ * get_pmt_positions(target, orientation) for a POM. */
+int pp_pom_pmt_directions(const pp_pom_model* model, float* dirs16x3);
+
+/* pmt_id_out[i] = 1..16 for a detected hit, 0 otherwise. rel_pos3: n x 3;
+ * wavelength (nm), photon_id and source_ix: n each (the photon's stream);
+ * wl_grid, acc1, acc2: model->n_wl each (acceptances of groups 1 and 2).
+ * *n_detected (may be NULL) counts the detected hits. Runs on `device`. */
+int pp_pom_pmt_hits(int device, const pp_pom_model* model, const float* wl_grid,
+                    const float* acc1, const float* acc2, int64_t n, const float* rel_pos3,
+                    const float* wavelength, const uint64_t* photon_id,
+                    const int32_t* source_ix, uint64_t seed, int32_t* pmt_id_out,
+                    int64_t* n_detected);
 
 /* Autotuner. The reference fixes its launch shape from the occupancy API
  * (src/photon_prop_cuda.jl:914-920); this library searches it instead.

@@ -79,6 +79,12 @@ def lib():
         L.ppo_adam.restype = None
         L.ppo_adam.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int] + \
             [ctypes.c_double] * 5
+        L.ppo_pom_pmt_dirs.restype = None
+        L.ppo_pom_pmt_dirs.argtypes = [c_void_p, c_void_p]
+        L.ppo_pom_pmt_hits.restype = None
+        L.ppo_pom_pmt_hits.argtypes = [c_void_p, c_float, c_float, c_int, c_void_p, c_void_p,
+                                       c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_uint64, c_void_p, c_void_p]
         L.ppo_sur_features.restype = None
         L.ppo_sur_features.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
         _lib = L
@@ -209,3 +215,27 @@ def sur_features(cascades, modules):
     out = np.zeros((c.shape[0], 8), np.float32)
     lib().ppo_sur_features(_p(c), _p(m), c.shape[0], _p(out))
     return out
+
+
+# ---- POM PMT detection (pp_pmt_oracle.c) ----
+
+def pom_pmt_dirs(orientation=None):
+    o = np.ascontiguousarray(np.eye(3) if orientation is None else orientation, np.float32)
+    d = np.zeros((16, 3), np.float64)
+    lib().ppo_pom_pmt_dirs(_p(o), _p(d))
+    return d
+
+
+def pom_pmt_hits(rel_pos, wl, pid, src, seed, wl_grid, acc1, acc2, sigma1, sigma2,
+                 orientation=None):
+    o = np.ascontiguousarray(np.eye(3) if orientation is None else orientation, np.float32)
+    rp = np.ascontiguousarray(rel_pos, np.float32)
+    w = np.ascontiguousarray(wl, np.float32)
+    pid = np.ascontiguousarray(pid, np.uint64)
+    src = np.ascontiguousarray(src, np.int32)
+    g, a1, a2 = (np.ascontiguousarray(v, np.float32) for v in (wl_grid, acc1, acc2))
+    out = np.zeros(len(w), np.int32)
+    ps = np.zeros(len(w), np.float64)
+    lib().ppo_pom_pmt_hits(_p(o), sigma1, sigma2, len(g), _p(g), _p(a1), _p(a2), len(w), _p(rp),
+                           _p(w), _p(pid), _p(src), ctypes.c_uint64(seed), _p(out), _p(ps))
+    return out, ps

@@ -1,0 +1,242 @@
+// pp_pmt.hip - PMT-level detection of the POM (include/photonprop_hip.h,
+// pp_pom_pmt_hits): check_pmt_hit(::POM{<:POMAcceptance}), src/pom.jl:246-305,
+// one lane per photon hit. The 16 PMT directions and both group widths are
+// kernel arguments (SGPRs); the acceptance tables sit in LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "photonprop_hip.h"
+#include "pp_device.h"
+
+void pp_set_error_message(const char* msg);
+
+namespace {
+
+constexpr int kPmts = 16;
+constexpr int kMaxWl = 1024;
+// PMT group of PMT j (src/pom.jl:172-174): PMTs 0-3 and 8-11 (25 deg from the
+// equator) are group 1, 4-7 and 12-15 (57.5 deg) group 2: index (j >> 2) & 1
+
+int fail(int code, const std::string& msg) {
+    pp_set_error_message(msg.c_str());
+    return code;
+}
+
+#define PP_HIP(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(PP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct PmtArgs {
+    float dir[kPmts][3];
+    float inv_s2[2];  // 1 / sigma^2 per group
+    int32_t n_wl;
+    const float* tab;  // wl | acc1 | acc2, n_wl each
+    const float* rel_pos;
+    const float* wl;
+    const uint64_t* pid;
+    const int32_t* src;
+    int64_t n;
+    uint32_t k0, k1;
+    int32_t* out;
+    unsigned long long* n_det;
+};
+
+// linear interpolation in a sorted table, 0 outside (extrapolation_bc = 0)
+__device__ __forceinline__ float interp0(const float* x, const float* y, int n, float v) {
+    if (!(v >= x[0] && v <= x[n - 1])) return 0.0f;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    const float t = (v - x[lo]) / (x[hi] - x[lo]);
+    return y[lo] + t * (y[hi] - y[lo]);
+}
+
+__global__ void __launch_bounds__(256) pom_pmt_kernel(PmtArgs a) {
+    __shared__ float tab[3 * kMaxWl];
+    for (int i = threadIdx.x; i < 3 * a.n_wl; i += blockDim.x) tab[i] = a.tab[i];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool det = false;
+    if (i < a.n) {
+        const float px = a.rel_pos[3 * i], py = a.rel_pos[3 * i + 1], pz = a.rel_pos[3 * i + 2];
+        const float wl = a.wl[i];
+        const float acc[2] = {interp0(tab, tab + a.n_wl, a.n_wl, wl),
+                              interp0(tab, tab + 2 * a.n_wl, a.n_wl, wl)};
+        float p[kPmts], sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kPmts; ++j) {
+            const int g = (j >> 2) & 1;
+            const float c = fminf(fmaxf(px * a.dir[j][0] + py * a.dir[j][1] + pz * a.dir[j][2], -1.0f),
+                                  1.0f);
+            const float pt = acosf(c);
+            const float st = sinf(pt);
+            // Rayleigh pdf over the density 0.5 sin(pt) of a uniform flux
+            const float pdf = pt * a.inv_s2[g] * __expf(-0.5f * pt * pt * a.inv_s2[g]);
+            p[j] = st != 0.0f ? pdf / (0.5f * st) * acc[g] * 0.125f : 0.0f;
+            sum += p[j];
+        }
+        const uint64_t id = a.pid[i];
+        const uint4 r = pp::philox(make_uint4(PP_CTR_PMT, (uint32_t)id, (uint32_t)(id >> 32),
+                                              (uint32_t)a.src[i]),
+                                   a.k0, a.k1);
+        int pmt = 0;
+        if (pp::u01(r.x) < sum) {
+            // categorical draw: first j whose cumulative weight exceeds u * sum
+            const float u = pp::u01(r.y) * sum;
+            float cum = 0.0f;
+            for (int j = 0; j < kPmts; ++j) {
+                cum += p[j];
+                if (p[j] > 0.0f) {
+                    pmt = j + 1;  // the last PMT with weight if rounding leaves u >= cum
+                    if (u < cum) break;
+                }
+            }
+            det = true;
+        }
+        a.out[i] = pmt;
+    }
+    const uint64_t m = __ballot(det);
+    if (__lane_id() == 0 && m) atomicAdd(a.n_det, (unsigned long long)__popcll(m));
+}
+
+// PMT (theta, phi) of the POM, src/pom.jl:108-128
+void pom_angles(double th[kPmts], double ph[kPmts]) {
+    const double d2r = M_PI / 180.0;
+    for (int k = 0; k < 4; ++k) {
+        th[k] = (90.0 - 25.0) * d2r;
+        ph[k] = k * M_PI / 2;
+        th[4 + k] = (90.0 - 57.5) * d2r;
+        ph[4 + k] = M_PI / 4 + k * M_PI / 2;
+        th[8 + k] = (90.0 + 25.0) * d2r;
+        th[12 + k] = (90.0 + 57.5) * d2r;
+    }
+    const double lower2[4] = {M_PI / 2, 0.0, 3 * M_PI / 2, M_PI};
+    const double lower[4] = {M_PI / 4, 7 * M_PI / 4, 5 * M_PI / 4, 3 * M_PI / 4};
+    for (int k = 0; k < 4; ++k) {
+        ph[8 + k] = lower2[k];
+        ph[12 + k] = lower[k];
+    }
+}
+
+int validate(const pp_pom_model* m) {
+    if (!m) return fail(PP_ERR_INVALID, "pom: NULL model");
+    if (!(m->sigma_grp1 > 0.0f && m->sigma_grp2 > 0.0f) || !std::isfinite(m->sigma_grp1) ||
+        !std::isfinite(m->sigma_grp2))
+        return fail(PP_ERR_INVALID, "pom: sigmas must be finite and > 0");
+    for (float v : m->orientation)
+        if (!std::isfinite(v)) return fail(PP_ERR_INVALID, "pom: non-finite orientation");
+    if (m->n_wl < 2 || m->n_wl > kMaxWl)
+        return fail(PP_ERR_INVALID, "pom: n_wl must be in [2, 1024]");
+    return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_pom_pmt_directions(const pp_pom_model* m, float* out) {
+    if (int rc = validate(m)) return rc;
+    if (!out) return fail(PP_ERR_INVALID, "pp_pom_pmt_directions: NULL output");
+    double th[kPmts], ph[kPmts];
+    pom_angles(th, ph);
+    for (int j = 0; j < kPmts; ++j) {
+        const double v[3] = {std::sin(th[j]) * std::cos(ph[j]), std::sin(th[j]) * std::sin(ph[j]),
+                             std::cos(th[j])};
+        // R takes e_z to e_x (rotation by +90 deg about e_y): (x, y, z) -> (z, y, -x)
+        const double w[3] = {v[2], v[1], -v[0]};
+        for (int r = 0; r < 3; ++r)
+            out[3 * j + r] = (float)(m->orientation[3 * r] * w[0] + m->orientation[3 * r + 1] * w[1] +
+                                     m->orientation[3 * r + 2] * w[2]);
+    }
+    return PP_OK;
+}
+
+int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, const float* acc1,
+                    const float* acc2, int64_t n, const float* rel_pos3, const float* wavelength,
+                    const uint64_t* photon_id, const int32_t* source_ix, uint64_t seed,
+                    int32_t* pmt_id_out, int64_t* n_detected) {
+    if (int rc = validate(m)) return rc;
+    if (n < 0 || n > ((int64_t)1 << 31)) return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: n out of range");
+    if (n_detected) *n_detected = 0;
+    if (n == 0) return PP_OK;
+    if (!wl_grid || !acc1 || !acc2 || !rel_pos3 || !wavelength || !photon_id || !source_ix ||
+        !pmt_id_out)
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: NULL array");
+    for (int i = 1; i < m->n_wl; ++i)
+        if (!(wl_grid[i] > wl_grid[i - 1]))
+            return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: wavelength grid must increase");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: no such HIP device");
+    PP_HIP(hipSetDevice(device));
+    (void)hipGetLastError();
+    PmtArgs a{};
+    float dirs[kPmts * 3];
+    if (int rc = pp_pom_pmt_directions(m, dirs)) return rc;
+    for (int j = 0; j < kPmts; ++j)
+        for (int r = 0; r < 3; ++r) a.dir[j][r] = dirs[3 * j + r];
+    a.inv_s2[0] = 1.0f / (m->sigma_grp1 * m->sigma_grp1);
+    a.inv_s2[1] = 1.0f / (m->sigma_grp2 * m->sigma_grp2);
+    a.n_wl = m->n_wl;
+    a.n = n;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    const size_t nw = (size_t)m->n_wl;
+    // one allocation: table | rel_pos | wl | pid | src | out | counter
+    const size_t off_pos = (3 * nw * 4 + 255) & ~(size_t)255;
+    const size_t off_wl = off_pos + (((size_t)n * 12 + 255) & ~(size_t)255);
+    const size_t off_pid = off_wl + (((size_t)n * 4 + 255) & ~(size_t)255);
+    const size_t off_src = off_pid + (((size_t)n * 8 + 255) & ~(size_t)255);
+    const size_t off_out = off_src + (((size_t)n * 4 + 255) & ~(size_t)255);
+    const size_t off_cnt = off_out + (((size_t)n * 4 + 255) & ~(size_t)255);
+    const size_t bytes = off_cnt + 8;
+    char* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PP_ERR_NOMEM, "pp_pom_pmt_hits: device allocation failed");
+    }
+    int rc = PP_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == PP_OK) rc = fail(PP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    std::vector<float> tab(3 * nw);
+    for (size_t k = 0; k < nw; ++k) {
+        tab[k] = wl_grid[k];
+        tab[nw + k] = acc1[k];
+        tab[2 * nw + k] = acc2[k];
+    }
+    chk(hipMemcpy(d, tab.data(), 3 * nw * 4, hipMemcpyHostToDevice), "upload table");
+    chk(hipMemcpy(d + off_pos, rel_pos3, (size_t)n * 12, hipMemcpyHostToDevice), "upload positions");
+    chk(hipMemcpy(d + off_wl, wavelength, (size_t)n * 4, hipMemcpyHostToDevice), "upload wavelengths");
+    chk(hipMemcpy(d + off_pid, photon_id, (size_t)n * 8, hipMemcpyHostToDevice), "upload ids");
+    chk(hipMemcpy(d + off_src, source_ix, (size_t)n * 4, hipMemcpyHostToDevice), "upload sources");
+    chk(hipMemset(d + off_cnt, 0, 8), "clear counter");
+    if (rc == PP_OK) {
+        a.tab = (const float*)d;
+        a.rel_pos = (const float*)(d + off_pos);
+        a.wl = (const float*)(d + off_wl);
+        a.pid = (const uint64_t*)(d + off_pid);
+        a.src = (const int32_t*)(d + off_src);
+        a.out = (int32_t*)(d + off_out);
+        a.n_det = (unsigned long long*)(d + off_cnt);
+        hipLaunchKernelGGL(pom_pmt_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a);
+        chk(hipGetLastError(), "pom_pmt_kernel launch");
+        chk(hipMemcpy(pmt_id_out, d + off_out, (size_t)n * 4, hipMemcpyDeviceToHost), "copy ids");
+        unsigned long long cnt = 0;
+        chk(hipMemcpy(&cnt, d + off_cnt, 8, hipMemcpyDeviceToHost), "copy count");
+        if (n_detected) *n_detected = (int64_t)cnt;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+}  // extern "C"

@@ -104,6 +104,14 @@ SUR_TIME_SHIFT = 20.0
 SUR_LABEL_FLOATS = 2 + SUR_BINS
 
 
+class pp_pom_model(Structure):
+    _fields_ = [("sigma_grp1", c_float), ("sigma_grp2", c_float), ("orientation", c_float * 9),
+                ("n_wl", c_int32), ("reserved", c_float * 4)]
+
+
+assert ctypes.sizeof(pp_pom_model) == 64
+
+
 class pp_cascade(Structure):
     _fields_ = [("position", c_float * 3), ("direction", c_float * 3), ("energy", c_float),
                 ("log_yield", c_float)]
@@ -145,6 +153,10 @@ SIGNATURES = [
     ("pp_search_info", c_int, [c_void_p, POINTER(c_int32), POINTER(c_float), POINTER(c_int32),
                                POINTER(c_int32)]),
     ("pp_work_counts", c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
+    ("pp_pom_pmt_directions", c_int, [POINTER(pp_pom_model), c_void_p]),
+    ("pp_pom_pmt_hits", c_int, [c_int, POINTER(pp_pom_model), c_void_p, c_void_p, c_void_p,
+                                c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_uint64,
+                                c_void_p, POINTER(c_int64)]),
     ("pp_device_info", c_int, [c_int, ctypes.c_char_p, c_int, POINTER(c_int32)]),
     ("pp_get_config", c_int, [c_void_p, POINTER(pp_config)]),
     ("pp_set_config", c_int, [c_void_p, POINTER(pp_config)]),

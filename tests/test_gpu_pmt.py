@@ -1,0 +1,67 @@
+"""PMT-level detection on the GPU (pp_pom_pmt_hits) against the oracle, and
+end to end after a propagation with POM modules."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pom(P):
+    from photonprop import pom
+    return pom
+
+
+def test_pmt_ids_match_oracle(O, pom):
+    acc = pom.default_acceptance()
+    rng = np.random.default_rng(3)
+    n = 1_000_000
+    rel = rng.normal(size=(n, 3))
+    rel /= np.linalg.norm(rel, axis=1, keepdims=True)
+    wl = rng.uniform(280, 820, n)
+    pid = rng.integers(0, 1 << 40, n).astype(np.uint64)
+    src = rng.integers(0, 500, n).astype(np.int32)
+    orient = np.array([[0.36, 0.48, -0.8], [-0.8, 0.6, 0.0], [0.48, 0.64, 0.6]])
+    m = pom.POM(np.zeros(3), 1)
+    g = pom._pmt_ids(rel, wl, acc, orient, pid, src, 77, 0)
+    o, psum = O.pom_pmt_hits(rel, wl, pid, src, 77, acc.wavelengths, acc.acc_1, acc.acc_2,
+                             acc.sigma_1, acc.sigma_2, orientation=orient)
+    diff = np.nonzero(g != o)[0]
+    print(f"{len(diff)} of {n} hits differ; detected {np.mean(g > 0):.4f}")
+    # a decision may flip only where a uniform lies within float rounding of a
+    # threshold (u_hit vs sum p, or a cumulative PMT weight)
+    assert len(diff) <= 20
+    np.testing.assert_allclose(pom.get_pmt_positions(m, orient), O.pom_pmt_dirs(orient),
+                               atol=2e-7)
+    # a photon's PMT does not depend on the batch it is in
+    g2 = pom._pmt_ids(rel[::7], wl[::7], acc, orient, pid[::7], src[::7], 77, 0)
+    assert np.array_equal(g2, g[::7])
+
+
+def test_pmt_hits_after_propagation(P, pom):
+    """README geometry with a POM: photons -> PMT hits; the detected fraction
+    equals the mean summed PMT probability of the photons' positions and
+    wavelengths, and hits concentrate on PMTs facing the source."""
+    target = pom.POM(np.float32([0, 0, 10]), 1)
+    src = P.PointlikeIsotropicEmitter(np.float32([0, 0, 0]), 0.0, 20_000_000)
+    medium = P.CascadiaMediumProperties()
+    spec = P.workloads.acceptance_weighted_spectrum(medium)
+    setup = P.PhotonPropSetup([src], [target], medium, spec, 1)
+    hbc, hbg = P.make_hit_buffers(capacity=1 << 22)
+    with hbg:
+        df = P.propagate_photons(setup, hbc, hbg, 15, vector_columns=False, with_ids=True)
+    assert len(df) > 5000
+    hits = pom.make_hits_from_photons(df, [target], seed=3)
+    frac = len(hits) / len(df)
+    acc = target.acceptance
+    print(f"{len(df)} photons on the module, {len(hits)} PMT hits ({frac:.4f})")
+    assert 0.005 < frac < 0.2
+    assert hits["pmt_id"].between(1, 16).all()
+    # PMT facing the source (-z side): the most hit PMTs point towards -z
+    d = pom.get_pmt_positions(target)
+    counts = np.bincount(hits["pmt_id"], minlength=17)[1:]
+    top = np.argsort(-counts)[:3]
+    assert np.all(d[top, 2] < 0), (counts, d[:, 2])
+    pe = pom.calc_pe_weight(hits.copy(), [target])
+    assert np.all(pe["qe_weight"] >= 0) and np.all(pe["qe_weight"] <= 1.0 + 1e-9)
+    del acc
