@@ -65,3 +65,21 @@ def test_pmt_hits_after_propagation(P, pom):
     pe = pom.calc_pe_weight(hits.copy(), [target])
     assert np.all(pe["qe_weight"] >= 0) and np.all(pe["qe_weight"] <= 1.0 + 1e-9)
     del acc
+
+
+def test_pmt_count_table(P, tmp_path):
+    """The surrogate-table writer: Arrow rows with 16 per-PMT weighted counts
+    from propagation -> PMT hits -> QE weights."""
+    import pyarrow as pa
+    from photonprop import tables
+    out = tmp_path / "tab.arrow"
+    rows = tables.pmt_count_table(out, 3, mode="em_shower", energy=300.0, dist=15.0,
+                                  randomize=False, seed=5, log=print)
+    with pa.memory_map(str(out)) as f:
+        t = pa.ipc.open_file(f).read_all()
+    assert t.num_rows == 3
+    assert set(t.column_names) == {"sim_ix", "pos", "dir", "pos_t", "pos_phi", "dir_t",
+                                   "dir_phi", "hits", "energy", "dist"}
+    hits = np.array(t.column("hits").to_pylist())
+    assert hits.shape == (3, 16) and np.all(hits >= 0) and hits.sum() > 0
+    np.testing.assert_allclose(np.linalg.norm(np.array(rows["pos"]), axis=1), 15.0, rtol=1e-6)
