@@ -74,8 +74,8 @@ typedef struct pp_source {
  *   Kopelevich scattering length, MixedHGES phase function, Quan-Fry
  *   dispersion. SIMPLE: SimpleMediumProperties (src/simple_medium.jl:3-15),
  *   constant lengths and an HG phase function. */
-enum pp_medium_kind { PP_MEDIUM_SIMPLE = 0, PP_MEDIUM_CASCADIA = 1 };
-enum pp_phase_kind { PP_PHASE_HG = 0, PP_PHASE_MIXED_HG_ES = 1 };
+enum pp_medium_kind { PP_MEDIUM_SIMPLE = 0, PP_MEDIUM_CASCADIA = 1, PP_MEDIUM_ICE = 2 };
+enum pp_phase_kind { PP_PHASE_HG = 0, PP_PHASE_MIXED_HG_ES = 1, PP_PHASE_MIXED_HG_SL = 2 };
 
 typedef struct pp_medium {
     int32_t kind;          /* enum pp_medium_kind                     */
@@ -95,7 +95,12 @@ typedef struct pp_medium {
     float salinity;        /* permille                                */
     float temperature;     /* deg C                                   */
     float pressure;        /* atm                                     */
-    float reserved[9];
+    /* ICE: HomogenousIceProperties (src/ice_properties.jl:26-177):
+     *   scattering length = sca_scale (1 - hg_g) / (b_dust_400 (wl/400)^-alpha),
+     *   phase/group index polynomials of the ice; phase function MIXED_HG_SL */
+    float ice_b_dust_400;     /* 1/m                                  */
+    float ice_alpha_sca_dust;
+    float reserved[7];
 } pp_medium; /* 96 bytes */
 
 /* ---- targets ----------------------------------------------------------
@@ -341,7 +346,9 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *   PP_FN_PHILOX          in (c0..c3,k0,k1 as u32 bits) out (r0..r3 as u32 bits)
  *   PP_FN_GAMMA           in (k, theta, key_lo, key_hi, pid_lo, src as u32 bits) out (x)
  *   PP_FN_SPECTRUM        in (u)                 out (wl)  (uses the context's spectrum)
- * The medium for PP_FN_CASCADIA is the context's medium. */
+ *   PP_FN_SL_COS          in (g, u)              out (cos)  simplified Liu
+ * PP_FN_CASCADIA evaluates the context's medium (any kind): scattering
+ * length, group velocity and phase index at wl. */
 enum pp_fn_id {
     PP_FN_ISOTROPIC_DIR = 0,
     PP_FN_ROT_FROM_EZ = 1,
@@ -355,7 +362,8 @@ enum pp_fn_id {
     PP_FN_PHILOX = 9,
     PP_FN_GAMMA = 10,
     PP_FN_SPECTRUM = 11,
-    PP_FN_COUNT = 12
+    PP_FN_SL_COS = 12,
+    PP_FN_COUNT = 13
 };
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out,
                int out_stride, int64_t n);

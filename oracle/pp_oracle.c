@@ -155,6 +155,15 @@ static double es_cos_d(double b, double u) {
 }
 float ppo_es_cos(float b, float u) { return (float)es_cos_d(b, u); }
 
+/* Simplified Liu phase function of PPC (mixed_hg_sl_scattering_func_ppc of
+ * CherenkovMediumBase, restated from PPC's published sampler):
+ * p(cos) ~ (1 + cos)^a, a = 2g/(1-g), whose mean is g; inverse CDF
+ * cos = 2 u^(1/(a+1)) - 1 = 2 u^((1-g)/(1+g)) - 1. */
+static double sl_cos_d(double g, double u) {
+    return clamp1d(2.0 * pow(u, (1.0 - g) / (1.0 + g)) - 1.0);
+}
+float ppo_sl_cos(float g, float u) { return (float)sl_cos_d(g, u); }
+
 /* MixedHGES(g, b, hg_fraction): HG with probability hg_fraction, else ES.
  * Draw order: branch choice, then the cosine. */
 float ppo_mixed_hges_cos(float g, float b, float hg_fraction, float u_choice, float u) {
@@ -238,6 +247,18 @@ void ppo_medium_values(const pp_medium* m, float wl, float* sca_len, float* c_gr
         *n_phase = m->phase_refractive_index;
         return;
     }
+    if (m->kind == PP_MEDIUM_ICE) {
+        /* HomogenousIceProperties, src/ice_properties.jl:105-126, :156-168 */
+        const double x = wl / 1000.0;
+        const double np = 1.55749 - 1.57988 * x + 3.99993 * x * x - 4.68271 * x * x * x +
+                          2.09354 * x * x * x * x;
+        const double ng = np * (1 + 0.227106 - 0.954648 * x + 1.42568 * x * x - 0.711832 * x * x * x);
+        const double eff = (double)m->ice_b_dust_400 * pow(wl / 400.0, -(double)m->ice_alpha_sca_dust);
+        *sca_len = (float)(1.0 / (eff / (1.0 - (double)m->hg_g)) * m->sca_scale);
+        *c_grp = (float)(PPO_C_VAC_M_NS / ng);
+        *n_phase = (float)np;
+        return;
+    }
     *sca_len = ppo_kopelevich_sca_len(wl, m->vol_conc_small, m->vol_conc_large, m->sca_scale);
     *c_grp = (float)ppo_quan_fry_group_velocity(wl, m->salinity, m->temperature, m->pressure);
     *n_phase = (float)ppo_quan_fry_n(wl, m->salinity, m->temperature, m->pressure);
@@ -247,6 +268,8 @@ void ppo_medium_values(const pp_medium* m, float wl, float* sca_len, float* c_gr
  * (src/simple_medium.jl:14), MixedHGES for Cascadia (src/water_properties.jl:57) */
 float ppo_phase_cos(const pp_medium* m, float u_choice, float u) {
     if (m->phase_kind == PP_PHASE_HG) return ppo_hg_cos(m->hg_g, u);
+    if (m->phase_kind == PP_PHASE_MIXED_HG_SL)
+        return u_choice < m->hg_fraction ? ppo_hg_cos(m->hg_g, u) : ppo_sl_cos(m->hg_g, u);
     return ppo_mixed_hges_cos(m->hg_g, m->es_b, m->hg_fraction, u_choice, u);
 }
 
@@ -641,6 +664,7 @@ int ppo_eval_fn(int fn, const pp_medium* m, const ppo_spectrum* sp, const float*
             if (!sp) return -1;
             y[0] = ppo_spectrum_sample(sp, x[0]);
             break;
+        case PP_FN_SL_COS: y[0] = ppo_sl_cos(x[0], x[1]); break;
         default: return -1;
         }
     }

@@ -57,6 +57,7 @@ void ppo_isotropic_dir(float u1, float u2, float out[3]);
 void ppo_rot_from_ez(const float a[3], const float b[3], float out[3]);
 float ppo_hg_cos(float g, float u);
 float ppo_es_cos(float b, float u);
+float ppo_sl_cos(float g, float u);
 float ppo_mixed_hges_cos(float g, float b, float hg_fraction, float u_choice, float u);
 void ppo_update_direction(const float dir[3], float cos_sca_theta, float u_phi, float out[3]);
 int ppo_intersect_sphere(const float c[3], float r, const float pos[3], const float dir[3],

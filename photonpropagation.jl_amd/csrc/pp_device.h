@@ -153,6 +153,8 @@ struct Phase {
     float hg_A, hg_B, hg_C, hg_D0, hg_D1;
     // ES: q = Q (1 - 2u); x = sgn (W - P3/W), W = cbrt(|q|/2 + sqrt(q^2/4 + P3^3))
     float es_Q, es_P3, es_P3cube;
+    // simplified Liu: p(cos) ~ (1 + cos)^(2g/(1-g)), cos = 2 u^((1-g)/(1+g)) - 1
+    float sl_exp;
 };
 
 __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg_fraction) {
@@ -167,6 +169,7 @@ __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg
     p.es_Q = (3.0f + b) / b;
     p.es_P3 = 1.0f / b;
     p.es_P3cube = p.es_P3 * p.es_P3 * p.es_P3;
+    p.sl_exp = (1.0f - g) / (1.0f + g);
     return p;
 }
 
@@ -185,8 +188,14 @@ __device__ __forceinline__ float es_cos(const Phase& p, float u) {
     return clamp1(q > 0.0f ? -x : x);
 }
 
+// simplified Liu (PPC's ice phase function): inverse CDF 2 u^e - 1, e = (1-g)/(1+g)
+__device__ __forceinline__ float sl_cos(const Phase& p, float u) {
+    return clamp1(2.0f * exp2f(p.sl_exp * log2f(u)) - 1.0f);
+}
+
 __device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float u) {
     if (p.kind == PP_PHASE_HG) return hg_cos(p, u);
+    if (p.kind == PP_PHASE_MIXED_HG_SL) return u_choice < p.hg_fraction ? hg_cos(p, u) : sl_cos(p, u);
     return u_choice < p.hg_fraction ? hg_cos(p, u) : es_cos(p, u);
 }
 
@@ -233,6 +242,8 @@ struct Medium {
     float kop1, kop2, kop3, sca_scale;
     // Quan-Fry: n = a0 + a1/wl + a2/wl^2 + a3/wl^3 ; n_g = a0 + 2a1/wl + 3a2/wl^2 + 4a3/wl^3
     float qf0, qf1, qf2, qf3;
+    // ICE: sca_len = ice_pref (wl/400)^ice_alpha, ice_pref = sca_scale (1-g) / b_dust_400
+    float ice_pref, ice_alpha;
 };
 
 struct MediumValues {
@@ -246,6 +257,16 @@ __device__ __forceinline__ MediumValues medium_values(const Medium& m, float wl)
         v.sca_len = m.sca_len;
         v.inv_c_grp = 1.0f / m.c_grp;
         n = m.n_phase;
+    } else if (m.kind == PP_MEDIUM_ICE) {
+        // src/ice_properties.jl:105-126 (indices), :164-168 (scattering)
+        v.sca_len = m.ice_pref * exp2f(m.ice_alpha * log2f(wl * (1.0f / 400.0f)));
+        // the index polynomials cancel to ~1.3: evaluated in double (once per
+        // photon) so that n and the group velocity keep float accuracy
+        const double x = (double)wl * 1e-3;
+        const double nd = 1.55749 + x * (-1.57988 + x * (3.99993 + x * (-4.68271 + x * 2.09354)));
+        const double ng = nd * (1.227106 + x * (-0.954648 + x * (1.42568 + x * -0.711832)));
+        n = (float)nd;
+        v.inv_c_grp = (float)(ng * (1.0 / (double)kCVacMNs));
     } else {
         const float lx = log2f(wl * (1.0f / 550.0f));
         const float b = m.kop1 * exp2f(-4.3f * lx) + m.kop2 * exp2f(-1.7f * lx) +

@@ -507,6 +507,11 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
     case PP_FN_SPECTRUM:
         y[0] = pp::spectrum_sample(e.spec, e.spec_n, e.spec_map, e.spec_mono, x[0]);
         break;
+    case PP_FN_SL_COS: {
+        pp::Phase ph = pp::make_phase(PP_PHASE_MIXED_HG_SL, x[0], 1.0f, 0.0f);
+        y[0] = pp::sl_cos(ph, x[1]);
+        break;
+    }
     default:
         break;
     }
@@ -547,6 +552,10 @@ pp::Medium lower_medium(const pp_medium& m) {
     d.qf1 = (float)(15.868 + 0.01155 * S - 0.00423 * T);
     d.qf2 = (float)(-4382.0);
     d.qf3 = (float)(1.1455e6);
+    if (m.kind == PP_MEDIUM_ICE) {
+        d.ice_pref = (float)((double)m.sca_scale * (1.0 - (double)m.hg_g) / (double)m.ice_b_dust_400);
+        d.ice_alpha = m.ice_alpha_sca_dust;
+    }
     return d;
 }
 
@@ -643,9 +652,14 @@ int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
         PP_HIP(hipMalloc(&c->d_cell_items, t.items.size() * 2));
         c->item_cap = t.items.size();
     }
-    PP_HIP(hipMemcpy(c->d_cells, t.cells.data(), t.cells.size() * sizeof(float4),
-                     hipMemcpyHostToDevice));
-    PP_HIP(hipMemcpy(c->d_cell_items, t.items.data(), t.items.size() * 2, hipMemcpyHostToDevice));
+    // uploads are ordered on the context stream, where the kernels that read
+    // them run (a blocking-stream copy from pageable memory may still be in
+    // flight when it returns); the sync keeps the host tables alive until done
+    PP_HIP(hipMemcpyAsync(c->d_cells, t.cells.data(), t.cells.size() * sizeof(float4),
+                          hipMemcpyHostToDevice, c->stream));
+    PP_HIP(hipMemcpyAsync(c->d_cell_items, t.items.data(), t.items.size() * 2,
+                          hipMemcpyHostToDevice, c->stream));
+    PP_HIP(hipStreamSynchronize(c->stream));
     c->grid = t.g;
     c->grid_cell_built = cell_req;
     c->grid_cell_z_built = cell_z_req;
@@ -686,19 +700,16 @@ int max_resident_blocks(pp_ctx* c, const void* fn, int block, size_t lds, int64_
 
 // One launch per source, as src/photon_prop_cuda.jl:925-944.
 int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
-                      const std::vector<uint64_t>& nph, bool grid, bool hg, bool count, size_t lds) {
+                      const std::vector<uint64_t>& nph, bool grid, int phase, bool count, size_t lds) {
     using KernelFn = void (*)(KParams);
-    // [count][grid][hg]
-    static const KernelFn kKernels[2][2][2] = {
-        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, false>,
-          propagate_kernel<PP_PHASE_HG, kSearchBrute, false>},
-         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, false>,
-          propagate_kernel<PP_PHASE_HG, kSearchGrid, false>}},
-        {{propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, true>,
-          propagate_kernel<PP_PHASE_HG, kSearchBrute, true>},
-         {propagate_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, true>,
-          propagate_kernel<PP_PHASE_HG, kSearchGrid, true>}}};
-    const KernelFn kern = kKernels[count][grid][hg];
+#define PP_K(PH, S, C) propagate_kernel<PH, S, C>
+#define PP_K3(S, C) {PP_K(PP_PHASE_HG, S, C), PP_K(PP_PHASE_MIXED_HG_ES, S, C), PP_K(PP_PHASE_MIXED_HG_SL, S, C)}
+    // [count][grid][phase kind]
+    static const KernelFn kKernels[2][2][3] = {{PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false)},
+                                               {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true)}};
+#undef PP_K3
+#undef PP_K
+    const KernelFn kern = kKernels[count][grid][phase];
     const int block = c->cfg.block_size;
     int64_t max_blocks = 0;
     if (int rc = max_resident_blocks(c, (const void*)kern, block, lds, max_blocks)) return rc;
@@ -721,19 +732,16 @@ int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
 // The whole source list in one persistent launch (propagate_batch_kernel);
 // with use_graph, as a hipGraph of counter clear + propagation.
 int launch_batch(pp_ctx* c, const KParams& kp, const pp_source* sources,
-                 const std::vector<uint64_t>& nph, bool grid, bool hg, bool count, size_t lds,
+                 const std::vector<uint64_t>& nph, bool grid, int phase, bool count, size_t lds,
                  bool use_graph) {
     using BatchFn = void (*)(KParams, BatchParams);
-    static const BatchFn kKernels[2][2][2] = {
-        {{propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, false>,
-          propagate_batch_kernel<PP_PHASE_HG, kSearchBrute, false>},
-         {propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, false>,
-          propagate_batch_kernel<PP_PHASE_HG, kSearchGrid, false>}},
-        {{propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchBrute, true>,
-          propagate_batch_kernel<PP_PHASE_HG, kSearchBrute, true>},
-         {propagate_batch_kernel<PP_PHASE_MIXED_HG_ES, kSearchGrid, true>,
-          propagate_batch_kernel<PP_PHASE_HG, kSearchGrid, true>}}};
-    const BatchFn kern = kKernels[count][grid][hg];
+#define PP_K(PH, S, C) propagate_batch_kernel<PH, S, C>
+#define PP_K3(S, C) {PP_K(PP_PHASE_HG, S, C), PP_K(PP_PHASE_MIXED_HG_ES, S, C), PP_K(PP_PHASE_MIXED_HG_SL, S, C)}
+    static const BatchFn kKernels[2][2][3] = {{PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false)},
+                                              {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true)}};
+#undef PP_K3
+#undef PP_K
+    const BatchFn kern = kKernels[count][grid][phase];
     const int block = c->cfg.block_size;
     const size_t n = nph.size();
     uint64_t total_ph = 0;
@@ -957,10 +965,16 @@ int pp_destroy(pp_ctx* ctx) {
 
 int pp_set_medium(pp_ctx* ctx, const pp_medium* m) {
     if (!ctx || !m) return fail(PP_ERR_INVALID, "pp_set_medium: NULL argument");
-    if (m->kind != PP_MEDIUM_SIMPLE && m->kind != PP_MEDIUM_CASCADIA)
+    if (m->kind != PP_MEDIUM_SIMPLE && m->kind != PP_MEDIUM_CASCADIA && m->kind != PP_MEDIUM_ICE)
         return fail(PP_ERR_INVALID, "pp_set_medium: unknown medium kind");
-    if (m->phase_kind != PP_PHASE_HG && m->phase_kind != PP_PHASE_MIXED_HG_ES)
+    if (m->phase_kind != PP_PHASE_HG && m->phase_kind != PP_PHASE_MIXED_HG_ES &&
+        m->phase_kind != PP_PHASE_MIXED_HG_SL)
         return fail(PP_ERR_INVALID, "pp_set_medium: unknown phase function kind");
+    if (m->phase_kind == PP_PHASE_MIXED_HG_SL && !(m->hg_fraction >= 0.0f && m->hg_fraction <= 1.0f))
+        return fail(PP_ERR_INVALID, "pp_set_medium: hg_fraction in [0,1] required");
+    if (m->kind == PP_MEDIUM_ICE &&
+        !(m->ice_b_dust_400 > 0.0f && std::isfinite(m->ice_alpha_sca_dust) && m->sca_scale > 0.0f))
+        return fail(PP_ERR_INVALID, "pp_set_medium: ICE needs b_dust_400 > 0, sca_scale > 0");
     if (!(m->hg_g > 0.0f && m->hg_g < 1.0f))
         return fail(PP_ERR_INVALID, "pp_set_medium: hg_g must be in (0,1)");
     if (m->phase_kind == PP_PHASE_MIXED_HG_ES &&
@@ -1000,7 +1014,9 @@ int pp_set_spectrum(pp_ctx* ctx, float mono_wl, const float* table, int n, int m
         PP_HIP(hipMalloc(&ctx->d_spec, (size_t)n * sizeof(float)));
         ctx->spec_cap = n;
     }
-    PP_HIP(hipMemcpy(ctx->d_spec, table, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpyAsync(ctx->d_spec, table, (size_t)n * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
     ctx->spec_n = n;
     ctx->spec_map = mapping;
     ctx->has_spectrum = true;
@@ -1030,8 +1046,9 @@ int pp_set_targets(pp_ctx* ctx, const pp_target* t, int n) {
         ctx->tgt_cap = n;
     }
     if (n > 0)
-        PP_HIP(hipMemcpy(ctx->d_targets, h.data(), (size_t)n * sizeof(float4),
-                         hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpyAsync(ctx->d_targets, h.data(), (size_t)n * sizeof(float4),
+                              hipMemcpyHostToDevice, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
     ctx->n_targets = n;
     ctx->h_targets.assign(h.begin(), h.begin() + n);
     ctx->h_radius.resize((size_t)n);
@@ -1090,7 +1107,7 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
                                  grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
     if (lds > (size_t)kMaxLdsBytes)
         return fail(PP_ERR_INVALID, "pp_propagate: targets and tables exceed the LDS budget");
-    const bool hg = ctx->medium.phase.kind == PP_PHASE_HG;
+    const int phase = ctx->medium.phase.kind;
     const bool count = ctx->cfg.count_work != 0;
     ctx->last_search = search;
     KParams kp{};
@@ -1123,9 +1140,9 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     ctx->n_launch = 0;
     const int mode = ctx->cfg.batch_mode == PP_BATCH_AUTO ? PP_BATCH_PERSISTENT : ctx->cfg.batch_mode;
     if (mode == PP_BATCH_PER_SOURCE) {
-        if (int rc = launch_per_source(ctx, kp, sources, nph, grid, hg, count, lds)) return rc;
+        if (int rc = launch_per_source(ctx, kp, sources, nph, grid, phase, count, lds)) return rc;
     } else {
-        if (int rc = launch_batch(ctx, kp, sources, nph, grid, hg, count, lds, mode == PP_BATCH_GRAPH))
+        if (int rc = launch_batch(ctx, kp, sources, nph, grid, phase, count, lds, mode == PP_BATCH_GRAPH))
             return rc;
     }
     PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1288,6 +1305,7 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
     // records to read: [lo, lo+cnt) of the device buffer
     const int64_t lo = ordered ? 0 : first, cnt = ordered ? ctx->n_hits : n;
     std::vector<HitRecord> rec((size_t)cnt);
+    PP_HIP(hipStreamSynchronize(ctx->stream));
     PP_HIP(hipMemcpy(rec.data(), ctx->hits.rec + lo, (size_t)cnt * sizeof(HitRecord),
                      hipMemcpyDeviceToHost));
     const uint64_t pid_mask = kMaxPhotonsPerSource - 1;
@@ -1326,8 +1344,8 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
 
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, int out_stride,
                int64_t n) {
-    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1};
-    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1};
+    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1, 2};
+    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1, 1};
     if (!ctx) return fail(PP_ERR_INVALID, "pp_eval_fn: NULL ctx");
     if (fn < 0 || fn >= PP_FN_COUNT) return fail(PP_ERR_INVALID, "pp_eval_fn: unknown function");
     if (n < 0 || (n > 0 && (!in || !out)))
@@ -1349,8 +1367,9 @@ int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, 
     }
     int rc = PP_OK;
     do {
-        if (hipMemcpy(d_in, in, in_b, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(PP_ERR_HIP, "copy in"); break; }
-        if (hipMemset(d_out, 0, out_b) != hipSuccess) { rc = fail(PP_ERR_HIP, "memset"); break; }
+        // everything on the context stream, in order
+        if (hipMemcpyAsync(d_in, in, in_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) { rc = fail(PP_ERR_HIP, "copy in"); break; }
+        if (hipMemsetAsync(d_out, 0, out_b, ctx->stream) != hipSuccess) { rc = fail(PP_ERR_HIP, "memset"); break; }
         EvalParams e{};
         e.fn = fn;
         e.in_stride = in_stride;
@@ -1366,8 +1385,8 @@ int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, 
         const int64_t grid = (n + 255) / 256;
         hipLaunchKernelGGL(eval_fn_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, e);
         if (hipGetLastError() != hipSuccess) { rc = fail(PP_ERR_HIP, "eval launch"); break; }
+        if (hipMemcpyAsync(out, d_out, out_b, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) { rc = fail(PP_ERR_HIP, "copy out"); break; }
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) { rc = fail(PP_ERR_HIP, "eval sync"); break; }
-        if (hipMemcpy(out, d_out, out_b, hipMemcpyDeviceToHost) != hipSuccess) { rc = fail(PP_ERR_HIP, "copy out"); break; }
     } while (0);
     (void)hipFree(d_in);
     (void)hipFree(d_out);

@@ -17,7 +17,8 @@ from .lightyield import (AxiconeEmitter, CherenkovEmitter, CherenkovTrackEmitter
                          PencilEmitter, PointlikeCherenkovEmitter, PointlikeIsotropicEmitter,
                          PointlikeTimeRangeEmitter, cascade_cherenkov_track_length,
                          rescale_source, split_source, total_lightyield)
-from .medium import (CascadiaMediumProperties, MediumProperties, SimpleMediumProperties,
+from .medium import (CascadiaMediumProperties, HomogenousIceProperties, MediumProperties,
+                     SimpleMediumProperties, make_homogenous_clearice_properties,
                      make_simple_medium)
 from .processing import (DeviceHitBuffer, HostHitBuffer, PhotonPropOOMException,
                          calc_total_weight, hits_to_dataframe, make_hit_buffers,

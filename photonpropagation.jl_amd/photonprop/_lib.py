@@ -37,8 +37,10 @@ SRC_CHERENKOV_TRACK = 9
 
 MEDIUM_SIMPLE = 0
 MEDIUM_CASCADIA = 1
+MEDIUM_ICE = 2
 PHASE_HG = 0
 PHASE_MIXED_HG_ES = 1
+PHASE_MIXED_HG_SL = 2
 SHAPE_SPHERE = 0
 SPEC_KNOTS = 0
 SPEC_TEXTURE = 1
@@ -53,9 +55,9 @@ BATCH_GRAPH = 3
 
 # enum pp_fn_id
 FN = dict(ISOTROPIC_DIR=0, ROT_FROM_EZ=1, HG_COS=2, ES_COS=3, UPDATE_DIR=4, SPHERE_ISEC=5,
-          STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11)
+          STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11, SL_COS=12)
 FN_ARITY = {0: (2, 3), 1: (6, 3), 2: (2, 1), 3: (2, 1), 4: (5, 3), 5: (11, 2), 6: (2, 1),
-            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1)}
+            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1), 12: (2, 1)}
 
 
 class pp_source(Structure):
@@ -73,7 +75,8 @@ class pp_medium(Structure):
                 ("sca_len", c_float), ("group_velocity", c_float),
                 ("phase_refractive_index", c_float), ("vol_conc_small", c_float),
                 ("vol_conc_large", c_float), ("salinity", c_float), ("temperature", c_float),
-                ("pressure", c_float), ("reserved", c_float * 9)]
+                ("pressure", c_float), ("ice_b_dust_400", c_float),
+                ("ice_alpha_sca_dust", c_float), ("reserved", c_float * 7)]
 
 
 class pp_target(Structure):

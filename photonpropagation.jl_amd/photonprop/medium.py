@@ -196,6 +196,89 @@ class SimpleMediumProperties(MediumProperties):
         return m
 
 
+def ice_phase_refractive_index(wl):
+    """src/ice_properties.jl:105-108 (wl in nm)."""
+    x = np.asarray(wl, np.float64) / 1000.0
+    return 1.55749 - 1.57988 * x + 3.99993 * x**2 - 4.68271 * x**3 + 2.09354 * x**4
+
+
+def ice_group_refractive_index(wl):
+    """src/ice_properties.jl:117-121."""
+    x = np.asarray(wl, np.float64) / 1000.0
+    return ice_phase_refractive_index(wl) * (1 + 0.227106 - 0.954648 * x + 1.42568 * x**2
+                                             - 0.711832 * x**3)
+
+
+@dataclass
+class HomogenousIceProperties(MediumProperties):
+    """HomogenousIceProperties (src/ice_properties.jl:26-177): SPICE-model
+    homogeneous ice; phase function: HG + simplified Liu (PPC)."""
+    radiation_length_: float
+    mean_scattering_angle_: float
+    hg_fraction: float
+    A_SPICE: float
+    B_SPICE: float
+    D_SPICE: float
+    E_SPICE: float
+    a_dust_400: float
+    b_dust_400: float
+    alpha_sca_dust: float
+    kappa_abs_dust: float
+    deltaTSPICE: float
+    abs_scale: float = 1.0
+    sca_scale: float = 1.0
+
+    def phase_refractive_index(self, wl):
+        return ice_phase_refractive_index(wl)
+
+    def group_velocity(self, wl):
+        return C_VAC_M_NS / ice_group_refractive_index(wl)
+
+    def cherenkov_angle(self, wl):
+        return np.arccos(1.0 / self.phase_refractive_index(wl))
+
+    def absorption_length(self, wl):  # :137-157
+        wl = np.asarray(wl, np.float64)
+        adust = self.a_dust_400 * (wl / 400.0) ** (-self.kappa_abs_dust)
+        atemp = self.A_SPICE * np.exp(-self.B_SPICE / wl) * (1 + 0.01 * self.deltaTSPICE)
+        return 1.0 / (adust + atemp) * self.abs_scale
+
+    def scattering_length(self, wl):  # :164-168
+        eff = self.b_dust_400 * (np.asarray(wl, np.float64) / 400.0) ** (-self.alpha_sca_dust)
+        return 1.0 / (eff / (1.0 - self.mean_scattering_angle_)) * self.sca_scale
+
+    def mean_scattering_angle(self):
+        return self.mean_scattering_angle_
+
+    def radiation_length(self):
+        return self.radiation_length_
+
+    def material_density(self):
+        return 0.910 * 1000
+
+    def to_pod(self):
+        m = _lib.pp_medium()
+        m.kind = _lib.MEDIUM_ICE
+        m.phase_kind = _lib.PHASE_MIXED_HG_SL
+        m.hg_g = self.mean_scattering_angle_
+        m.hg_fraction = self.hg_fraction
+        m.sca_scale = self.sca_scale
+        m.abs_scale = self.abs_scale
+        m.ice_b_dust_400 = self.b_dust_400
+        m.ice_alpha_sca_dust = self.alpha_sca_dust
+        return m
+
+
+def make_homogenous_clearice_properties(abs_scale=1.0, sca_scale=1.0):
+    """The homogeneous clear-ice model (src/ice_properties.jl:50-65)."""
+    return HomogenousIceProperties(
+        radiation_length_=39.652 / 0.9216, mean_scattering_angle_=0.9, hg_fraction=0.45,
+        A_SPICE=6954.090332031250, B_SPICE=6617.754394531250, D_SPICE=0.0, E_SPICE=0.0,
+        a_dust_400=0.006350013, b_dust_400=0.02331206666666667,
+        alpha_sca_dust=0.898608505726, kappa_abs_dust=1.084106802940,
+        deltaTSPICE=12.115215000000001, abs_scale=abs_scale, sca_scale=sca_scale)
+
+
 def make_simple_medium(absorption_length, scattering_length, phase_refractive_index,
                        mean_scattering_angle, group_velocity):
     """Keyword-style constructor mirroring Base.@kwdef SimpleMediumProperties."""

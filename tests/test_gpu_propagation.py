@@ -345,3 +345,20 @@ def test_autotune_picks_a_candidate_and_keeps_results(P, tmp_path):
         got = P._lib.pp_config()
         P._lib.load_library().pp_get_config(d2.handle, __import__("ctypes").byref(got))
         assert P.autotune.config_to_dict(got) == c
+
+
+def test_ice_medium_photon_parity(P, O, dev):
+    """Homogeneous clear ice (SPICE lengths, HG + simplified-Liu phase
+    function): photon by photon against the oracle."""
+    medium = P.make_homogenous_clearice_properties()
+    targets = P.make_detector_grid(3, 3, 15.0, 4, 10.0, radius=0.3)
+    src = P.PointlikeIsotropicEmitter(np.float32([2, 1, 3]), 0.0, 1_000_000)
+    spec = P.make_cuda_spectral_dist(P.make_cherenkov_spectrum((300.0, 700.0), medium).spectral_dist,
+                                     (300.0, 700.0), 600)
+    wl = P.workloads.Workload("ice", [src], targets, medium, spec, 21)
+    g, gsim = gpu_run(P, dev, wl)
+    r, rsim = oracle_run(P, O, wl)
+    assert gsim == rsim
+    n, d = compare_photon_by_photon(g, r)
+    print(f"ice: {n} common hits, {d} differing")
+    assert n > 100

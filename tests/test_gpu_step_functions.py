@@ -97,6 +97,23 @@ def test_es_cos(P, O, dev, b):
     check_vec(P, O, dev, "ES_COS", x, f"es_cos b={b}")
 
 
+@pytest.mark.parametrize("g_", [0.9, 0.5])
+def test_sl_cos(P, O, dev, g_):
+    u = RNG.random(200_000, dtype=np.float32)
+    x = np.stack([np.full_like(u, g_), u], 1)
+    check_vec(P, O, dev, "SL_COS", x, f"sl_cos g={g_}")
+
+
+def test_consecutive_evaluations_see_their_own_inputs(P, O, dev):
+    """Regression: inputs are uploaded on the context's stream, so a call
+    never reads the previous call's inputs left in a reused allocation."""
+    u = RNG.random(500_000, dtype=np.float32)
+    for g_ in (0.9, 0.2, 0.7, 0.45):
+        x = np.stack([np.full_like(u, g_), u], 1)
+        g, r = both(P, O, dev, "SL_COS", x)
+        assert np.max(np.abs(g - r)) < 1e-5, g_
+
+
 def test_update_direction(P, O, dev):
     n = 200_000
     x = np.concatenate([unit(n), RNG.uniform(-1, 1, (n, 1)).astype(np.float32),
@@ -152,6 +169,17 @@ def test_step_length(P, O, dev):
 
 def test_cascadia_medium_values(P, O, dev):
     m = P.CascadiaMediumProperties()
+    P.processing.stage(dev, [P.spherical_target((0, 0, 0), 1.0, 1)], m, P.Monochromatic(450.0))
+    wl = np.linspace(290.0, 810.0, 20_001, dtype=np.float32)
+    g, r = both(P, O, dev, "CASCADIA", wl[:, None], medium=m.to_pod())
+    for k, name in enumerate(("sca_len", "c_grp", "n_phase")):
+        err = np.abs(g[:, k] - r[:, k]) / ulp(r[:, k])
+        report(name, err)
+        assert err.max() <= 4, name
+
+
+def test_ice_medium_values(P, O, dev):
+    m = P.make_homogenous_clearice_properties(sca_scale=1.3)
     P.processing.stage(dev, [P.spherical_target((0, 0, 0), 1.0, 1)], m, P.Monochromatic(450.0))
     wl = np.linspace(290.0, 810.0, 20_001, dtype=np.float32)
     g, r = both(P, O, dev, "CASCADIA", wl[:, None], medium=m.to_pod())
