@@ -105,15 +105,27 @@ typedef struct pp_medium {
 
 /* ---- targets ----------------------------------------------------------
  * Index order is kept: it defines the reference's first-hit rule
- * (src/photon_prop_cuda.jl:497-506). */
-enum pp_shape_kind { PP_SHAPE_SPHERE = 0 };
+ * (src/photon_prop_cuda.jl:497-506).
+ * SPHERE     check_intersection(::Spherical)   jl:309-351: centre, radius.
+ * RECTANGLE  check_intersection(::Rectangular) jl:353-378: a horizontal
+ *            rectangle in the plane z = position[2], centred on position,
+ *            sides length[0] (x) and length[1] (y).
+ * DISC       check_intersection(::Circular)    jl:380-404: a horizontal disc
+ *            in the plane z = position[2], centre position, radius.
+ * Planar targets are crossed at d = (z_t - z) / dir_z with 0 <= d <= step
+ * and never when dir_z == 0. The reference declares dir_z > 0 through an
+ * unchecked `assume`; this library evaluates the same formula for either
+ * sign, so a plane is also crossed from above. A target list holding a
+ * planar target is searched by testing every target in index order (the
+ * reference's loop); the grid search takes spheres only. */
+enum pp_shape_kind { PP_SHAPE_SPHERE = 0, PP_SHAPE_RECTANGLE = 1, PP_SHAPE_DISC = 2 };
 
 typedef struct pp_target {
     float position[3];
-    float radius;
-    int32_t shape_kind;    /* enum pp_shape_kind      */
+    float radius;          /* SPHERE, DISC             */
+    int32_t shape_kind;    /* enum pp_shape_kind       */
     int32_t module_id;     /* carried, not used on device */
-    float reserved[2];
+    float length[2];       /* RECTANGLE: length_x, length_y */
 } pp_target; /* 32 bytes */
 
 /* ---- spectrum -----------------------------------------------------------
@@ -347,6 +359,9 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *   PP_FN_GAMMA           in (k, theta, key_lo, key_hi, pid_lo, src as u32 bits) out (x)
  *   PP_FN_SPECTRUM        in (u)                 out (wl)  (uses the context's spectrum)
  *   PP_FN_SL_COS          in (g, u)              out (cos)  simplified Liu
+ *   PP_FN_PLANE_ISEC      in (kind, cx,cy,cz, a, b, px,py,pz, dx,dy,dz, step) out (hit,d)
+ *                         kind as u32 bits (RECTANGLE: a, b = length_x, length_y;
+ *                         DISC: a = radius)                      jl:353-404
  * PP_FN_CASCADIA evaluates the context's medium (any kind): scattering
  * length, group velocity and phase index at wl. */
 enum pp_fn_id {
@@ -363,7 +378,8 @@ enum pp_fn_id {
     PP_FN_GAMMA = 10,
     PP_FN_SPECTRUM = 11,
     PP_FN_SL_COS = 12,
-    PP_FN_COUNT = 13
+    PP_FN_PLANE_ISEC = 13,
+    PP_FN_COUNT = 14
 };
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out,
                int out_stride, int64_t n);

@@ -18,6 +18,8 @@
  *   kopelevich      reference-held value 56.86782 m @ 488 nm (axicone.ipynb)
  *   intersect       closed-form hit point on the sphere, inside -> no hit,
  *                   brute-force first-hit order, no-scatter hit fraction
+ *   plane           crossing point on the plane, inside/outside the shape by
+ *                   construction, beams of known geometry through a disc
  *   isotropic       cos(theta) ~ U(-1,1), phi ~ U(0,2pi) (KS), unit norm
  *   hg / es / mixed moments and per-bin chi2 vs the analytic pdf
  *   rot_from_ez     R e_z = a, orthonormality, dot(R b, a) = b_z
@@ -199,6 +201,39 @@ int ppo_intersect_sphere(const float c[3], float r, const float pos[3], const fl
         return 1;
     }
     return 0;
+}
+
+/* check_intersection(::Rectangular / ::Circular)   src/photon_prop_cuda.jl:353-404.
+ *   No hit when dir_z == 0; d = (z_t - z) / dir_z, miss if d < 0 or d > step;
+ *   hit iff the crossing point is strictly inside the rectangle
+ *   (|x - x_t| < length_x/2, |y - y_t| < length_y/2) or the disc
+ *   ((x - x_t)^2 + (y - y_t)^2 < r^2, r^2 a Float32 value). The crossing is
+ *   evaluated exactly; d is rounded once. a = length_x or radius, b = length_y. */
+int ppo_intersect_plane(int kind, const float c[3], float a, float b, const float pos[3],
+                        const float dir[3], float step, float* d_out) {
+    if (dir[2] == 0.0f) return 0;
+    double d = ((double)c[2] - pos[2]) / dir[2];
+    if (d < 0.0 || d > (double)step) return 0;
+    double x = (double)pos[0] + (double)dir[0] * d - c[0];
+    double y = (double)pos[1] + (double)dir[1] * d - c[1];
+    int in;
+    if (kind == PP_SHAPE_RECTANGLE)
+        in = fabs(x) < (double)a / 2.0 && fabs(y) < (double)b / 2.0;
+    else
+        in = x * x + y * y < (double)(a * a);
+    if (!in) return 0;
+    *d_out = (float)d;
+    return 1;
+}
+
+/* one target of any shape: check_intersection   src/photon_prop_cuda.jl:345-404 */
+static int ppo_intersect_target(const pp_target* t, const float pos[3], const float dir[3],
+                                float step, float* d_out) {
+    if (t->shape_kind == PP_SHAPE_SPHERE)
+        return ppo_intersect_sphere(t->position, t->radius, pos, dir, step, d_out);
+    float a = t->shape_kind == PP_SHAPE_RECTANGLE ? t->length[0] : t->radius;
+    return ppo_intersect_plane(t->shape_kind, t->position, a, t->length[1], pos, dir, step,
+                               d_out);
 }
 
 /* step_size = -log(eta) * sca_len   src/photon_prop_cuda.jl:488-490 */
@@ -488,8 +523,7 @@ int ppo_transport(ppo_photon* st, const pp_medium* m, const pp_target* targets, 
         float dist_to_target = 0.0f;
         for (int ix = 0; ix < n_targets; ++ix) {
             float d;
-            if (ppo_intersect_sphere(targets[ix].position, targets[ix].radius, pos, dir,
-                                     step_size, &d)) {
+            if (ppo_intersect_target(&targets[ix], pos, dir, step_size, &d)) {
                 isec = 1;
                 module_ix = ix + 1;
                 dist_to_target = d;
@@ -633,6 +667,15 @@ int ppo_eval_fn(int fn, const pp_medium* m, const ppo_spectrum* sp, const float*
         case PP_FN_SPHERE_ISEC: {
             float d = 0.0f;
             int h = ppo_intersect_sphere(x, x[3], x + 4, x + 7, x[10], &d);
+            y[0] = h ? 1.0f : 0.0f;
+            y[1] = h ? d : 0.0f;
+            break;
+        }
+        case PP_FN_PLANE_ISEC: {
+            float d = 0.0f;
+            uint32_t kind;
+            memcpy(&kind, x, 4);
+            int h = ppo_intersect_plane((int)kind, x + 1, x[4], x[5], x + 6, x + 9, x[12], &d);
             y[0] = h ? 1.0f : 0.0f;
             y[1] = h ? d : 0.0f;
             break;

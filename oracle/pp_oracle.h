@@ -62,6 +62,8 @@ float ppo_mixed_hges_cos(float g, float b, float hg_fraction, float u_choice, fl
 void ppo_update_direction(const float dir[3], float cos_sca_theta, float u_phi, float out[3]);
 int ppo_intersect_sphere(const float c[3], float r, const float pos[3], const float dir[3],
                          float step, float* d_out);
+int ppo_intersect_plane(int kind, const float c[3], float a, float b, const float pos[3],
+                        const float dir[3], float step, float* d_out);
 float ppo_step_length(float eta, float sca_len);
 float ppo_kopelevich_sca_len(float wl, float vs, float vl, float sca_scale);
 double ppo_quan_fry_n(double wl, double S, double T, double P);

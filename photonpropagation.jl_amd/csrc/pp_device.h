@@ -74,6 +74,18 @@ __device__ __forceinline__ float rcp_nr(float x) {
     return fmaf(fmaf(-x, r, 1.0f), r, r);
 }
 
+// Square root by the hardware instruction (1 ulp). The correctly rounded
+// sequence the compiler emits for sqrtf adds two Newton corrections and a
+// denormal rescale (~14 instructions); every argument here is zero or a
+// normal number, and the per-step results only need a few ulp.
+__host__ __device__ __forceinline__ float sqrt_hw(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+
 // Cube root of x > 0: hardware log2/exp2 estimate, one Newton step.
 __device__ __forceinline__ float cbrt_pos(float x) {
     const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
@@ -121,7 +133,7 @@ __device__ __forceinline__ f3 dir_from_cos_phi(float ct, float st, float u_phi) 
 //   src/photon_prop_cuda.jl:73-77
 __device__ __forceinline__ f3 isotropic_dir(float u1, float u2) {
     const float ct = -1.0f + u1 * 2.0f;
-    const float st = sqrtf((1.0f - ct) * (1.0f + ct));
+    const float st = sqrt_hw((1.0f - ct) * (1.0f + ct));
     return dir_from_cos_phi(ct, st, u2);
 }
 
@@ -183,7 +195,7 @@ __device__ __forceinline__ float hg_cos(const Phase& p, float u) {
 __device__ __forceinline__ float es_cos(const Phase& p, float u) {
     const float q = p.es_Q * (1.0f - 2.0f * u);
     const float hq = 0.5f * fabsf(q);
-    const float w = cbrt_pos(hq + sqrtf(hq * hq + p.es_P3cube));
+    const float w = cbrt_pos(hq + sqrt_hw(hq * hq + p.es_P3cube));
     const float x = w - p.es_P3 * rcp_nr(w);
     return clamp1(q > 0.0f ? -x : x);
 }
@@ -201,7 +213,7 @@ __device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float
 
 // update_direction   src/photon_prop_cuda.jl:280-298
 __device__ __forceinline__ f3 update_direction(f3 dir, float ct, float u_phi) {
-    const float st = sqrtf(1.0f - ct * ct);
+    const float st = sqrt_hw(1.0f - ct * ct);
     return rot_from_ez(dir, dir_from_cos_phi(ct, st, u_phi));
 }
 
@@ -226,8 +238,25 @@ __host__ __device__ __forceinline__ bool sphere_hit(float4 t, f3 pos, f3 dir, fl
     const float a = dir.x * dx + dir.y * dy + dir.z * dz;
     const float qx = dx - a * dir.x, qy = dy - a * dir.y, qz = dz - a * dir.z;
     const float b = (a * excess) * a + (t.w - (qx * qx + qy * qy + qz * qz));  // t.w = r^2
-    d = (b >= 0.0f) ? (-a - sqrtf(b)) : -1.0f;
+    d = (b >= 0.0f) ? (-a - sqrt_hw(b)) : -1.0f;
     return (d > 0.0f) & (d < step);
+}
+// Planar targets: check_intersection(::Rectangular / ::Circular)
+//   src/photon_prop_cuda.jl:353-404. The plane z = t.z is crossed at
+// d = (t.z - z) / dir_z (correctly rounded division, as the reference's
+// Float32 `/`), a hit iff dir_z != 0, 0 <= d <= step and the crossing point
+// lies strictly inside the shape: |x - t.x| < length_x/2 and
+// |y - t.y| < length_y/2 for a rectangle (e = (_, lx/2, ly/2, _)),
+// (x - t.x)^2 + (y - t.y)^2 < r^2 for a disc (t.w = r^2).
+__host__ __device__ __forceinline__ bool plane_hit(uint32_t kind, float4 t, float4 e, f3 pos,
+                                                   f3 dir, float step, float& d) {
+    d = -1.0f;
+    if (dir.z == 0.0f) return false;
+    d = (t.z - pos.z) / dir.z;
+    if (d < 0.0f || d > step) return false;
+    const float x = (pos.x + dir.x * d) - t.x, y = (pos.y + dir.y * d) - t.y;
+    return kind == PP_SHAPE_RECTANGLE ? (fabsf(x) < e.y && fabsf(y) < e.z)
+                                      : (x * x + y * y < t.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,7 +307,7 @@ __device__ __forceinline__ MediumValues medium_values(const Medium& m, float wl)
         v.inv_c_grp = ng * (1.0f / kCVacMNs);
     }
     v.cos_c = 1.0f / n;  // cherenkov_angle: acos(1/n), beta = 1
-    v.sin_c = sqrtf((1.0f - v.cos_c) * (1.0f + v.cos_c));
+    v.sin_c = sqrt_hw((1.0f - v.cos_c) * (1.0f + v.cos_c));
     return v;
 }
 
@@ -290,7 +319,7 @@ __device__ __forceinline__ f3 cherenkov_track_dir(float u1, float u2) {
     constexpr float kI = 0.9672917f;  // 1 - exp(-2.61 * 2^0.39)
     const float t = -logf(1.0f - u1 * kI) * kInvB;
     const float ct = fmaxf(1.0f - powf(t, kInvA), -1.0f);
-    const float st = sqrtf((1.0f - ct) * (1.0f + ct));
+    const float st = sqrt_hw((1.0f - ct) * (1.0f + ct));
     return dir_from_cos_phi(ct, st, u2);
 }
 
@@ -305,7 +334,7 @@ __device__ __forceinline__ float normal_sample(const Stream& s, uint32_t sub) {
     const uint4 r = s.block(sampler_word(sub, 0));
     float sp, cp;
     sincosf(u01(r.y) * kTwoPi, &sp, &cp);
-    return sqrtf(-2.0f * logf(u01_open0(r.x))) * cp;
+    return sqrt_hw(-2.0f * logf(u01_open0(r.x))) * cp;
 }
 
 // Marsaglia-Tsang gamma(shape, scale). Each sampler block: a Box-Muller pair
@@ -322,7 +351,7 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
     const float c = 1.0f / sqrtf(9.0f * d);
     for (uint32_t i = 0; i < PP_GAMMA_MAX_BLOCKS; ++i) {
         const uint4 r = s.block(sampler_word(sub, i));
-        const float rad = sqrtf(-2.0f * logf(u01_open0(r.x)));
+        const float rad = sqrt_hw(-2.0f * logf(u01_open0(r.x)));
         float sp, cp;
         sincosf(u01(r.y) * kTwoPi, &sp, &cp);
         const float z[2] = {rad * cp, rad * sp};

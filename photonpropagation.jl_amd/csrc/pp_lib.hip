@@ -107,7 +107,7 @@ __device__ __forceinline__ f3 cherenkov_dir(const pp_source& src, const pp::Medi
         const float gx = pp::gamma_sample(src.beta_a, 1.0f, s, PP_SUB_BETA_X);
         const float gy = pp::gamma_sample(src.beta_b, 1.0f, s, PP_SUB_BETA_Y);
         const float tc = (gx / (gx + gy)) * 2.0f - 1.0f;
-        const float ts = sqrtf((1.0f - tc) * (1.0f + tc));
+        const float ts = pp::sqrt_hw((1.0f - tc) * (1.0f + tc));
         ph = pp::rot_from_ez(pp::dir_from_cos_phi(tc, ts, pp::u01(x.y)), ph);
     }
     const f3 axis = pp::make3(src.direction[0], src.direction[1], src.direction[2]);
@@ -142,13 +142,13 @@ __device__ __forceinline__ PhotonInit init_photon(const KParams& p, const pp_sou
     }
     case PP_SRC_COLLIMATED_ISOTROPIC: {
         const float ct = src.cut_angle_cos + pp::u01(r.z) * (1.0f - src.cut_angle_cos);
-        st.dir = pp::dir_from_cos_phi(ct, sqrtf((1.0f - ct) * (1.0f + ct)), pp::u01(r.y));
+        st.dir = pp::dir_from_cos_phi(ct, pp::sqrt_hw((1.0f - ct) * (1.0f + ct)), pp::u01(r.y));
         break;
     }
     case PP_SRC_PENCIL:
         if (src.beam_divergence > 0.0f) {
             const float theta =
-                sqrtf(pp::u01(r.y) * (src.beam_divergence * src.beam_divergence));
+                pp::sqrt_hw(pp::u01(r.y) * (src.beam_divergence * src.beam_divergence));
             float sn, cs;
             sincosf(theta, &sn, &cs);
             st.dir = pp::rot_from_ez(axis, pp::dir_from_cos_phi(cs, sn, pp::u01(r.z)));
@@ -225,7 +225,7 @@ __device__ __forceinline__ bool store_hits(bool has_hit, const HitRec& h, uint32
 // ---------------------------------------------------------------------------
 using pp::lds_bytes;
 
-enum { kSearchBrute = 0, kSearchGrid = 1 };
+enum { kSearchBrute = 0, kSearchGrid = 1, kSearchMixed = 2 };
 
 // LDS tables of one workgroup: targets, grid cells and items, spectrum knots
 struct Lds {
@@ -239,11 +239,13 @@ template <int SEARCH>
 __device__ __forceinline__ Lds stage_lds(const KParams& p) {
     extern __shared__ float4 lds[];
     Lds l;
+    // a mixed (sphere and planar) list has two records per target
+    const int n_rec = SEARCH == kSearchMixed ? 2 * p.n_targets : p.n_targets;
     l.tgt = lds;
-    l.cells = lds + p.n_targets;
+    l.cells = lds + n_rec;
     l.items = reinterpret_cast<uint16_t*>(l.cells + (SEARCH == kSearchGrid ? 2 * p.grid.n_cells : 0));
     l.spec = reinterpret_cast<float*>(l.items + (SEARCH == kSearchGrid ? ((p.grid.n_items + 1) & ~1) : 0));
-    for (int i = threadIdx.x; i < p.n_targets; i += blockDim.x) l.tgt[i] = p.targets[i];
+    for (int i = threadIdx.x; i < n_rec; i += blockDim.x) l.tgt[i] = p.targets[i];
     if (SEARCH == kSearchGrid) {
         for (int i = threadIdx.x; i < 2 * p.grid.n_cells; i += blockDim.x) l.cells[i] = p.cells[i];
         for (int i = threadIdx.x; i < p.grid.n_items; i += blockDim.x) l.items[i] = p.cell_items[i];
@@ -254,12 +256,16 @@ __device__ __forceinline__ Lds stage_lds(const KParams& p) {
 }
 
 template <int SEARCH>
-using SearchT = typename std::conditional<SEARCH == kSearchGrid, pp::GridSearch, pp::BruteSearch>::type;
+using SearchT = typename std::conditional<
+    SEARCH == kSearchGrid, pp::GridSearch,
+    typename std::conditional<SEARCH == kSearchMixed, pp::MixedSearch, pp::BruteSearch>::type>::type;
 
 template <int SEARCH>
 __device__ __forceinline__ SearchT<SEARCH> make_search(const KParams& p, const Lds& l) {
     if constexpr (SEARCH == kSearchGrid) {
         return pp::GridSearch{l.tgt, l.cells, l.items, p.grid};
+    } else if constexpr (SEARCH == kSearchMixed) {
+        return pp::MixedSearch{l.tgt, p.n_targets};
     } else {
         return pp::BruteSearch{l.tgt, p.n_targets};
     }
@@ -512,6 +518,17 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         y[0] = pp::sl_cos(ph, x[1]);
         break;
     }
+    case PP_FN_PLANE_ISEC: {
+        float d;
+        const uint32_t kind = bits(x[0]);
+        const float4 t = make_float4(x[1], x[2], x[3], x[4] * x[4]);
+        const float4 ext = make_float4(x[0], x[4] / 2.0f, x[5] / 2.0f, 0.0f);
+        const bool h = pp::plane_hit(kind, t, ext, pp::make3(x[6], x[7], x[8]),
+                                     pp::make3(x[9], x[10], x[11]), x[12], d);
+        y[0] = h ? 1.0f : 0.0f;
+        y[1] = h ? d : 0.0f;
+        break;
+    }
     default:
         break;
     }
@@ -587,6 +604,7 @@ struct pp_ctx {
     int tgt_cap = 0;
     std::vector<float4> h_targets;  // x, y, z, r^2
     std::vector<float> h_radius;
+    bool planar = false;  // the list holds rectangles or discs: two records per target
     // uniform grid over the targets, rebuilt when targets or cell size change
     pp::GridDesc grid{};
     float grid_cell_built = -1.0f;  // requested cell size of the built grid
@@ -700,16 +718,17 @@ int max_resident_blocks(pp_ctx* c, const void* fn, int block, size_t lds, int64_
 
 // One launch per source, as src/photon_prop_cuda.jl:925-944.
 int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
-                      const std::vector<uint64_t>& nph, bool grid, int phase, bool count, size_t lds) {
+                      const std::vector<uint64_t>& nph, int skind, int phase, bool count, size_t lds) {
     using KernelFn = void (*)(KParams);
 #define PP_K(PH, S, C) propagate_kernel<PH, S, C>
 #define PP_K3(S, C) {PP_K(PP_PHASE_HG, S, C), PP_K(PP_PHASE_MIXED_HG_ES, S, C), PP_K(PP_PHASE_MIXED_HG_SL, S, C)}
-    // [count][grid][phase kind]
-    static const KernelFn kKernels[2][2][3] = {{PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false)},
-                                               {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true)}};
+    // [count][search kind][phase kind]
+    static const KernelFn kKernels[2][3][3] = {
+        {PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false), PP_K3(kSearchMixed, false)},
+        {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true), PP_K3(kSearchMixed, true)}};
 #undef PP_K3
 #undef PP_K
-    const KernelFn kern = kKernels[count][grid][phase];
+    const KernelFn kern = kKernels[count][skind][phase];
     const int block = c->cfg.block_size;
     int64_t max_blocks = 0;
     if (int rc = max_resident_blocks(c, (const void*)kern, block, lds, max_blocks)) return rc;
@@ -732,16 +751,17 @@ int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
 // The whole source list in one persistent launch (propagate_batch_kernel);
 // with use_graph, as a hipGraph of counter clear + propagation.
 int launch_batch(pp_ctx* c, const KParams& kp, const pp_source* sources,
-                 const std::vector<uint64_t>& nph, bool grid, int phase, bool count, size_t lds,
+                 const std::vector<uint64_t>& nph, int skind, int phase, bool count, size_t lds,
                  bool use_graph) {
     using BatchFn = void (*)(KParams, BatchParams);
 #define PP_K(PH, S, C) propagate_batch_kernel<PH, S, C>
 #define PP_K3(S, C) {PP_K(PP_PHASE_HG, S, C), PP_K(PP_PHASE_MIXED_HG_ES, S, C), PP_K(PP_PHASE_MIXED_HG_SL, S, C)}
-    static const BatchFn kKernels[2][2][3] = {{PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false)},
-                                              {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true)}};
+    static const BatchFn kKernels[2][3][3] = {
+        {PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false), PP_K3(kSearchMixed, false)},
+        {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true), PP_K3(kSearchMixed, true)}};
 #undef PP_K3
 #undef PP_K
-    const BatchFn kern = kKernels[count][grid][phase];
+    const BatchFn kern = kKernels[count][skind][phase];
     const int block = c->cfg.block_size;
     const size_t n = nph.size();
     uint64_t total_ph = 0;
@@ -1026,33 +1046,63 @@ int pp_set_spectrum(pp_ctx* ctx, float mono_wl, const float* table, int n, int m
 int pp_set_targets(pp_ctx* ctx, const pp_target* t, int n) {
     if (!ctx) return fail(PP_ERR_INVALID, "pp_set_targets: NULL ctx");
     if (n < 0 || (n > 0 && !t)) return fail(PP_ERR_INVALID, "pp_set_targets: bad target list");
-    if ((size_t)n * sizeof(float4) + 8192 * sizeof(float) > (size_t)kMaxLdsBytes)
+    bool planar = false;
+    for (int i = 0; i < n; ++i) {
+        const pp_target& x = t[i];
+        const std::string who = "pp_set_targets: target " + std::to_string(i);
+        if (!finite3(x.position)) return fail(PP_ERR_INVALID, who + " has a non-finite position");
+        if (x.shape_kind == PP_SHAPE_SPHERE || x.shape_kind == PP_SHAPE_DISC) {
+            if (!(x.radius > 0.0f) || !std::isfinite(x.radius))
+                return fail(PP_ERR_INVALID, who + " needs a finite radius > 0");
+        } else if (x.shape_kind == PP_SHAPE_RECTANGLE) {
+            if (!(x.length[0] > 0.0f) || !(x.length[1] > 0.0f) || !std::isfinite(x.length[0]) ||
+                !std::isfinite(x.length[1]))
+                return fail(PP_ERR_INVALID, who + " needs finite side lengths > 0");
+        } else {
+            return fail(PP_ERR_INVALID, who + " has an unknown shape_kind");
+        }
+        planar |= x.shape_kind != PP_SHAPE_SPHERE;
+    }
+    // records per target: (x, y, z, r^2), plus (kind bits, lx/2, ly/2, 0) in a
+    // list with planar targets
+    const int per = planar ? 2 : 1;
+    if ((size_t)n * per * sizeof(float4) + 8192 * sizeof(float) > (size_t)kMaxLdsBytes)
         return fail(PP_ERR_INVALID, "pp_set_targets: too many targets for LDS staging");
     if (int rc = set_dev(ctx)) return rc;
-    std::vector<float4> h((size_t)std::max(n, 1));
+    std::vector<float4> h((size_t)std::max(n * per, 1));
     for (int i = 0; i < n; ++i) {
-        if (t[i].shape_kind != PP_SHAPE_SPHERE)
-            return fail(PP_ERR_INVALID, "pp_set_targets: only spherical targets are supported");
-        if (!finite3(t[i].position) || !(t[i].radius > 0.0f) || !std::isfinite(t[i].radius))
-            return fail(PP_ERR_INVALID, "pp_set_targets: target " + std::to_string(i) +
-                                            " has a non-finite position or radius <= 0");
-        h[i] = make_float4(t[i].position[0], t[i].position[1], t[i].position[2],
-                           t[i].radius * t[i].radius);
+        const pp_target& x = t[i];
+        // r^2 and the half lengths are Float32 values, as target_shape.radius^2
+        // and length_x/2 of the reference
+        const float w = x.shape_kind == PP_SHAPE_RECTANGLE ? 0.0f : x.radius * x.radius;
+        h[(size_t)i * per] = make_float4(x.position[0], x.position[1], x.position[2], w);
+        if (planar) {
+            uint32_t kind = (uint32_t)x.shape_kind;
+            float kf;
+            std::memcpy(&kf, &kind, 4);
+            h[(size_t)i * per + 1] = x.shape_kind == PP_SHAPE_RECTANGLE
+                                         ? make_float4(kf, x.length[0] / 2.0f, x.length[1] / 2.0f, 0.0f)
+                                         : make_float4(kf, 0.0f, 0.0f, 0.0f);
+        }
     }
-    if (n > ctx->tgt_cap) {
+    if (n * per > ctx->tgt_cap) {
         if (ctx->d_targets) PP_HIP(hipFree(ctx->d_targets));
         ctx->d_targets = nullptr;
-        PP_HIP(hipMalloc(&ctx->d_targets, (size_t)n * sizeof(float4)));
-        ctx->tgt_cap = n;
+        PP_HIP(hipMalloc(&ctx->d_targets, (size_t)n * per * sizeof(float4)));
+        ctx->tgt_cap = n * per;
     }
     if (n > 0)
-        PP_HIP(hipMemcpyAsync(ctx->d_targets, h.data(), (size_t)n * sizeof(float4),
+        PP_HIP(hipMemcpyAsync(ctx->d_targets, h.data(), (size_t)n * per * sizeof(float4),
                               hipMemcpyHostToDevice, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
     ctx->n_targets = n;
-    ctx->h_targets.assign(h.begin(), h.begin() + n);
+    ctx->planar = planar;
+    ctx->h_targets.resize((size_t)n);
     ctx->h_radius.resize((size_t)n);
-    for (int i = 0; i < n; ++i) ctx->h_radius[(size_t)i] = t[i].radius;
+    for (int i = 0; i < n; ++i) {
+        ctx->h_targets[(size_t)i] = h[(size_t)i * per];
+        ctx->h_radius[(size_t)i] = t[i].radius;
+    }
     ctx->grid_cell_built = -1.0f;  // grid is rebuilt on the next propagation
     return PP_OK;
 }
@@ -1091,19 +1141,25 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     }
     if (int rc = set_dev(ctx)) return rc;
 
-    const int block = ctx->cfg.block_size;
     // search: grid for more than a handful of targets unless asked otherwise
     int search = ctx->cfg.search;
-    if (search == PP_SEARCH_AUTO) search = ctx->n_targets > 8 ? PP_SEARCH_GRID : PP_SEARCH_BRUTE;
+    if (ctx->planar && search == PP_SEARCH_GRID)
+        return fail(PP_ERR_INVALID,
+                    "pp_propagate: the grid search takes spherical targets only (the list holds "
+                    "rectangles or discs)");
+    if (search == PP_SEARCH_AUTO)
+        search = ctx->n_targets > 8 && !ctx->planar ? PP_SEARCH_GRID : PP_SEARCH_BRUTE;
     if (ctx->n_targets == 0) search = PP_SEARCH_BRUTE;
     const bool grid = search == PP_SEARCH_GRID;
+    const int skind = grid ? kSearchGrid : ctx->planar ? kSearchMixed : kSearchBrute;
     if (grid) {
         const float want = ctx->cfg.grid_cell, want_z = ctx->cfg.grid_cell_z;
         if (ctx->grid_cell_built != want || ctx->grid_cell_z_built != want_z ||
             ctx->grid_cell_built < -0.5f)
             if (int rc = build_grid(ctx, want, want_z)) return rc;
     }
-    const size_t lds = lds_bytes(ctx->n_targets, grid ? ctx->grid.n_cells : 0,
+    const size_t lds = lds_bytes(ctx->planar ? 2 * ctx->n_targets : ctx->n_targets,
+                                 grid ? ctx->grid.n_cells : 0,
                                  grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
     if (lds > (size_t)kMaxLdsBytes)
         return fail(PP_ERR_INVALID, "pp_propagate: targets and tables exceed the LDS budget");
@@ -1140,9 +1196,9 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     ctx->n_launch = 0;
     const int mode = ctx->cfg.batch_mode == PP_BATCH_AUTO ? PP_BATCH_PERSISTENT : ctx->cfg.batch_mode;
     if (mode == PP_BATCH_PER_SOURCE) {
-        if (int rc = launch_per_source(ctx, kp, sources, nph, grid, phase, count, lds)) return rc;
+        if (int rc = launch_per_source(ctx, kp, sources, nph, skind, phase, count, lds)) return rc;
     } else {
-        if (int rc = launch_batch(ctx, kp, sources, nph, grid, phase, count, lds, mode == PP_BATCH_GRAPH))
+        if (int rc = launch_batch(ctx, kp, sources, nph, skind, phase, count, lds, mode == PP_BATCH_GRAPH))
             return rc;
     }
     PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1344,8 +1400,8 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
 
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, int out_stride,
                int64_t n) {
-    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1, 2};
-    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1, 1};
+    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1, 2, 13};
+    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1, 1, 2};
     if (!ctx) return fail(PP_ERR_INVALID, "pp_eval_fn: NULL ctx");
     if (fn < 0 || fn >= PP_FN_COUNT) return fail(PP_ERR_INVALID, "pp_eval_fn: unknown function");
     if (n < 0 || (n > 0 && (!in || !out)))

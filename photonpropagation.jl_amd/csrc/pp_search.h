@@ -5,7 +5,9 @@
 // ("first target in index order", not the nearest).
 //
 // BruteSearch tests every target (wave-uniform loop over LDS, broadcast
-// reads) - the reference's own algorithm.
+// reads) - the reference's own algorithm. MixedSearch is the same loop over
+// lists that hold planar targets (rectangles, discs); for those the hit
+// window is 0 <= d <= step (src/photon_prop_cuda.jl:353-404).
 // GridSearch walks the step segment through a uniform grid of box cells
 // (3-D DDA, Amanatides & Woo), cubic or taller along z. Each cell record holds the bounding box of
 // the targets overlapping the cell; a slab test of the segment against that
@@ -87,6 +89,38 @@ __host__ __device__ __forceinline__ uint32_t float_bits(float x) {
 #endif
 }
 
+// Target lists with planar shapes: every target in index order, as
+// BruteSearch, with the shape test chosen per target (the kind is the same
+// for all lanes, so the branch is wave-uniform). Two records per target:
+// (x, y, z, r^2) and (kind bits, length_x/2, length_y/2, 0).
+struct MixedSearch {
+    const float4* tgt;
+    int n;
+    __host__ __device__ __forceinline__ int operator()(f3 pos, f3 dir, float step, float& dt) const {
+        Work w;
+        return find<false>(pos, dir, step, dt, w);
+    }
+    template <bool COUNT>
+    __host__ __device__ __forceinline__ int find(f3 pos, f3 dir, float step, float& dt,
+                                                 Work& w) const {
+        if (COUNT) w.tests += (uint32_t)n;
+        const float ex = dir_excess(dir);
+        int found = -1;
+        for (int t = 0; t < n; ++t) {
+            const float4 a = tgt[2 * t], e = tgt[2 * t + 1];
+            const uint32_t kind = float_bits(e.x);
+            float d;
+            const bool h = kind == PP_SHAPE_SPHERE ? sphere_hit(a, pos, dir, ex, step, d)
+                                                   : plane_hit(kind, a, e, pos, dir, step, d);
+            if (h && found < 0) {
+                found = t;
+                dt = d;
+            }
+        }
+        return found;
+    }
+};
+
 struct GridSearch {
     const float4* tgt;
     const float4* cells;    // 2 per cell: (lo xyz, start bits), (hi xyz, count bits)
@@ -133,6 +167,10 @@ struct GridSearch {
         const int dcx = sx, dcy = sy * g.nx, dcz = sz * g.nx * g.ny;
         int c = (cz * g.ny + cy) * g.nx + cx;
         float4 lo = cells[2 * c], hi = cells[2 * c + 1];
+        // slab parameters of a cell box as one FMA per plane: (b - p) / d =
+        // b (1/d) - p (1/d); the rounding of p (1/d) moves a plane by about
+        // |p| 2^-24 along its axis, far inside the boxes' 1 cm margin
+        const float px_ = pos.x * ix_, py_ = pos.y * iy_, pz_ = pos.z * iz_;
         const int max_visits = g.nx + g.ny + g.nz;
         for (int v = 0; v < max_visits; ++v) {
             // next cell first, so that its record loads while this one is tested
@@ -149,9 +187,9 @@ struct GridSearch {
             const uint32_t cnt = float_bits(hi.w);
             if (COUNT) w.cells += 1;
             if (cnt != 0) {
-                const float ax0 = (lo.x - pos.x) * ix_, ax1 = (hi.x - pos.x) * ix_;
-                const float ay0 = (lo.y - pos.y) * iy_, ay1 = (hi.y - pos.y) * iy_;
-                const float az0 = (lo.z - pos.z) * iz_, az1 = (hi.z - pos.z) * iz_;
+                const float ax0 = fmaf(lo.x, ix_, -px_), ax1 = fmaf(hi.x, ix_, -px_);
+                const float ay0 = fmaf(lo.y, iy_, -py_), ay1 = fmaf(hi.y, iy_, -py_);
+                const float az0 = fmaf(lo.z, iz_, -pz_), az1 = fmaf(hi.z, iz_, -pz_);
                 const float b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)),
                                          fmaxf(fminf(az0, az1), 0.0f));
                 const float b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)),

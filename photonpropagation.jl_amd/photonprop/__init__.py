@@ -7,7 +7,8 @@ setup, emitter, target, medium and spectrum types and ``propagate_photons``
 with the reference's hit-table layout.
 """
 from . import _lib, autotune, workloads
-from .detection import (HomogeneousDetector, PhotonTarget, Spherical, make_detector_grid,
+from .detection import (Circular, HomogeneousDetector, PhotonTarget, Rectangular, Spherical,
+                        circular_target, make_detector_grid, rectangular_target,
                         spherical_target, targets_to_pod)
 from .lightyield import (AxiconeEmitter, CherenkovEmitter, CherenkovTrackEmitter,
                          CollimatedIsotropicEmitter, ExtendedCherenkovEmitter,

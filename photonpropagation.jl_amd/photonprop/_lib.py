@@ -42,6 +42,8 @@ PHASE_HG = 0
 PHASE_MIXED_HG_ES = 1
 PHASE_MIXED_HG_SL = 2
 SHAPE_SPHERE = 0
+SHAPE_RECTANGLE = 1
+SHAPE_DISC = 2
 SPEC_KNOTS = 0
 SPEC_TEXTURE = 1
 SEARCH_AUTO = 0
@@ -55,9 +57,11 @@ BATCH_GRAPH = 3
 
 # enum pp_fn_id
 FN = dict(ISOTROPIC_DIR=0, ROT_FROM_EZ=1, HG_COS=2, ES_COS=3, UPDATE_DIR=4, SPHERE_ISEC=5,
-          STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11, SL_COS=12)
+          STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11, SL_COS=12,
+          PLANE_ISEC=13)
 FN_ARITY = {0: (2, 3), 1: (6, 3), 2: (2, 1), 3: (2, 1), 4: (5, 3), 5: (11, 2), 6: (2, 1),
-            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1), 12: (2, 1)}
+            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1), 12: (2, 1),
+            13: (13, 2)}
 
 
 class pp_source(Structure):
@@ -81,7 +85,7 @@ class pp_medium(Structure):
 
 class pp_target(Structure):
     _fields_ = [("position", c_float * 3), ("radius", c_float), ("shape_kind", c_int32),
-                ("module_id", c_int32), ("reserved", c_float * 2)]
+                ("module_id", c_int32), ("length", c_float * 2)]
 
 
 class pp_config(Structure):

@@ -1,9 +1,12 @@
-"""Targets: spherical optical modules.
+"""Targets: spherical optical modules and planar (rectangular, circular)
+detectors.
 
 Mirror of the NeutrinoTelescopeBase target types the reference passes to the
-kernel: ``Spherical(position, radius)`` shapes wrapped in targets that carry a
-``module_id`` (src/processing.jl:57-70 reads ``targ.shape`` and
-``targ.module_id``).
+kernel: ``Spherical(position, radius)``, ``Rectangular(position, length_x,
+length_y)`` and ``Circular(position, radius)`` shapes wrapped in targets that
+carry a ``module_id`` (src/processing.jl:57-70 reads ``targ.shape`` and
+``targ.module_id``). Planar shapes lie in the horizontal plane through their
+position (check_intersection, src/photon_prop_cuda.jl:353-404).
 """
 from __future__ import annotations
 
@@ -24,8 +27,27 @@ class Spherical:
 
 
 @dataclass
+class Rectangular:
+    position: np.ndarray
+    length_x: float
+    length_y: float
+
+    def surface_area(self):
+        return float(self.length_x) * float(self.length_y)
+
+
+@dataclass
+class Circular:
+    position: np.ndarray
+    radius: float
+
+    def surface_area(self):
+        return np.pi * float(self.radius) ** 2
+
+
+@dataclass
 class PhotonTarget:
-    shape: Spherical
+    shape: Spherical | Rectangular | Circular
     module_id: int
 
 
@@ -44,17 +66,35 @@ def spherical_target(position, radius, module_id):
                         int(module_id))
 
 
+def rectangular_target(position, length_x, length_y, module_id):
+    return PhotonTarget(Rectangular(np.asarray(position, dtype=np.float32), float(length_x),
+                                    float(length_y)), int(module_id))
+
+
+def circular_target(position, radius, module_id):
+    return PhotonTarget(Circular(np.asarray(position, dtype=np.float32), float(radius)),
+                        int(module_id))
+
+
 def targets_to_pod(targets):
     """Lower a target list to the pp_target array (index order preserved)."""
     arr = (_lib.pp_target * max(len(targets), 1))()
     for i, t in enumerate(targets):
         shape = t.shape if hasattr(t, "shape") else t
-        if not isinstance(shape, Spherical):
-            raise TypeError(f"target {i}: only Spherical shapes are supported")
         pos = np.asarray(shape.position, dtype=np.float32).reshape(3)
         arr[i].position[:] = [float(x) for x in pos]
-        arr[i].radius = float(np.float32(shape.radius))
-        arr[i].shape_kind = _lib.SHAPE_SPHERE
+        if isinstance(shape, Spherical):
+            arr[i].radius = float(np.float32(shape.radius))
+            arr[i].shape_kind = _lib.SHAPE_SPHERE
+        elif isinstance(shape, Circular):
+            arr[i].radius = float(np.float32(shape.radius))
+            arr[i].shape_kind = _lib.SHAPE_DISC
+        elif isinstance(shape, Rectangular):
+            arr[i].length[:] = [float(np.float32(shape.length_x)),
+                                float(np.float32(shape.length_y))]
+            arr[i].shape_kind = _lib.SHAPE_RECTANGLE
+        else:
+            raise TypeError(f"target {i}: unknown shape {type(shape).__name__}")
         arr[i].module_id = int(getattr(t, "module_id", i + 1))
     return arr
 

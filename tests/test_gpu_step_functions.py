@@ -156,6 +156,50 @@ def test_sphere_intersection(P, O, dev):
     assert np.max(excess) <= 4
 
 
+@pytest.mark.parametrize("kind", [1, 2], ids=["rectangle", "disc"])
+def test_plane_intersection(P, O, dev, kind):
+    """check_intersection(::Rectangular / ::Circular): decisions agree except
+    for rays within 16 float ulps of a decision boundary (the shape's edge,
+    d = 0, d = step); d within 4 ulp + kappa."""
+    n = 200_000
+    c = RNG.uniform(-20, 20, (n, 3)).astype(np.float32)
+    a = RNG.uniform(0.5, 3.0, n).astype(np.float32)
+    b = RNG.uniform(0.5, 3.0, n).astype(np.float32)
+    cross = c + np.column_stack([RNG.uniform(-1.3, 1.3, (n, 2)) * np.column_stack([a, b]) *
+                                 (0.5 if kind == 1 else 1.0), np.zeros(n)]).astype(np.float32)
+    d = unit(n)
+    dist = RNG.uniform(-5, 40, n).astype(np.float32)
+    p = (cross - d * dist[:, None]).astype(np.float32)
+    step = RNG.exponential(30.0, n).astype(np.float32)
+    x = np.zeros((n, 13), np.float32)
+    x[:, 0] = np.array([kind], np.uint32).view(np.float32)[0]
+    x[:, 1:4], x[:, 4], x[:, 5], x[:, 6:9], x[:, 9:12], x[:, 12] = c, a, b, p, d, step
+    g, ref = both(P, O, dev, "PLANE_ISEC", x)
+    # exact crossing of the float inputs
+    cd, pd, dd = c.astype(np.float64), p.astype(np.float64), d.astype(np.float64)
+    t = (cd[:, 2] - pd[:, 2]) / dd[:, 2]
+    ex = pd[:, 0] + dd[:, 0] * t - cd[:, 0]
+    ey = pd[:, 1] + dd[:, 1] * t - cd[:, 1]
+    scale = np.abs(pd[:, :2]).max(1) + np.abs(cd[:, :2]).max(1) + np.abs(t) + a
+    eps = 16 * 2.0**-24
+    if kind == 1:
+        margin = np.minimum(np.abs(np.abs(ex) - a / 2.0), np.abs(np.abs(ey) - b / 2.0))
+    else:
+        rr = np.float32(a) * np.float32(a)
+        margin = np.abs(ex * ex + ey * ey - rr) / (2 * np.sqrt(rr) + 1e-30)
+    tz = np.abs(pd[:, 2]) + np.abs(cd[:, 2])
+    edge = (margin < eps * scale) | (np.abs(t) < eps * tz / np.abs(dd[:, 2])) | (
+        np.abs(t - step) < eps * (step + tz / np.abs(dd[:, 2])))
+    print(f"plane kind {kind}: {edge.mean():.2e} of rows within the decision band, "
+          f"hit rate {ref[:, 0].mean():.3f}")
+    assert np.all((g[:, 0] == ref[:, 0]) | edge)
+    h = (g[:, 0] == 1) & (ref[:, 0] == 1)
+    assert h.sum() > 20_000
+    err = np.abs(g[h, 1] - ref[h, 1]) / ulp(ref[h, 1])
+    report(f"plane d kind {kind}", err)
+    assert err.max() <= 4
+
+
 def test_step_length(P, O, dev):
     u = ((RNG.integers(0, 2**24, 200_000) + 1) * 2.0**-24).astype(np.float32)
     x = np.stack([u, RNG.uniform(5, 100, len(u)).astype(np.float32)], 1)
