@@ -40,7 +40,7 @@ inline float u32_as_float(uint32_t u) {
 // float rounding of the DDA walk). Each cell lists its targets in ascending
 // index order and keeps the union of their boxes, grown by a further
 // 1 cm + 1e-4 max(h, hz), so that a float slab test of a segment against it can
-// only err towards testing. cell_req <= 0 picks 1.5x the edge of one cubic
+// only err towards testing; an empty cell keeps a box at +inf. cell_req <= 0 picks 1.5x the edge of one cubic
 // cell per target (about 3.4 targets per cell); cell_z_req = 0 makes the cells
 // cubic, < 0 one layer over the whole height (columns: strings are vertical),
 // > 0 sets the z edge. The edges grow by 1.25x until the tables fit `budget`
@@ -124,6 +124,8 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         out.cells.assign(2 * (size_t)n_cells, float4{});
         for (int64_t k = 0; k < n_cells; ++k) {
             const size_t c = (size_t)k;
+            if (count[c] == 0)  // a box at +inf: every slab test against it fails
+                for (int a = 0; a < 3; ++a) box_lo[3 * c + a] = box_hi[3 * c + a] = INFINITY;
             out.cells[2 * c] = make_float4(box_lo[3 * c], box_lo[3 * c + 1], box_lo[3 * c + 2],
                                            u32_as_float(start[c]));
             out.cells[2 * c + 1] = make_float4(box_hi[3 * c], box_hi[3 * c + 1], box_hi[3 * c + 2],

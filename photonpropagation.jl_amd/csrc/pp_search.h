@@ -61,7 +61,7 @@ struct BruteSearch {
 };
 
 __host__ __device__ __forceinline__ float nonzero(float x) {
-    return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x;
+    return copysignf(fmaxf(fabsf(x), 1e-20f), x);
 }
 
 // Reciprocal for the grid walk: the hardware estimate on the device (the walk
@@ -138,11 +138,15 @@ struct GridSearch {
         // below come from these same values so that they agree with ix_..iz_
         const float ix_ = fast_rcp(nonzero(dir.x)), iy_ = fast_rcp(nonzero(dir.y)),
                     iz_ = fast_rcp(nonzero(dir.z));
+        // plane parameters as one FMA per plane: (b - p) / d = b (1/d) - p (1/d);
+        // the rounding of p (1/d) moves a plane by about |p| 2^-24 along its
+        // axis, far inside the binning margin and the boxes' 1 cm margin
+        const float px_ = pos.x * ix_, py_ = pos.y * iy_, pz_ = pos.z * iz_;
         // clip the segment [0, step] to the grid box
         const float ex = g.ox + g.nx * g.h, ey = g.oy + g.ny * g.h, ez = g.oz + g.nz * g.hz;
-        const float tx0 = (g.ox - pos.x) * ix_, tx1 = (ex - pos.x) * ix_;
-        const float ty0 = (g.oy - pos.y) * iy_, ty1 = (ey - pos.y) * iy_;
-        const float tz0 = (g.oz - pos.z) * iz_, tz1 = (ez - pos.z) * iz_;
+        const float tx0 = fmaf(g.ox, ix_, -px_), tx1 = fmaf(ex, ix_, -px_);
+        const float ty0 = fmaf(g.oy, iy_, -py_), ty1 = fmaf(ey, iy_, -py_);
+        const float tz0 = fmaf(g.oz, iz_, -pz_), tz1 = fmaf(ez, iz_, -pz_);
         const float t_in = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         const float t_out = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), step));
         int found = -1;
@@ -155,65 +159,54 @@ struct GridSearch {
         const int cz = clampi((int)floorf((pz - g.oz) * g.inv_hz), g.nz - 1);
         const int sx = ix_ > 0.0f ? 1 : -1, sy = iy_ > 0.0f ? 1 : -1, sz = iz_ > 0.0f ? 1 : -1;
         // ray parameter of the next cell boundary on each axis
-        float tmx = (g.ox + (float)(cx + (sx > 0)) * g.h - pos.x) * ix_;
-        float tmy = (g.oy + (float)(cy + (sy > 0)) * g.h - pos.y) * iy_;
-        float tmz = (g.oz + (float)(cz + (sz > 0)) * g.hz - pos.z) * iz_;
+        float tmx = fmaf(g.ox + (float)(cx + (sx > 0)) * g.h, ix_, -px_);
+        float tmy = fmaf(g.oy + (float)(cy + (sy > 0)) * g.h, iy_, -py_);
+        float tmz = fmaf(g.oz + (float)(cz + (sz > 0)) * g.hz, iz_, -pz_);
         const float tdx = g.h * fabsf(ix_), tdy = g.h * fabsf(iy_), tdz = g.hz * fabsf(iz_);
-        // cells left before leaving the grid along each axis, cell index and
-        // its increments: the walk advances with selects, not branches
-        int rx = sx > 0 ? g.nx - 1 - cx : cx;
-        int ry = sy > 0 ? g.ny - 1 - cy : cy;
-        int rz = sz > 0 ? g.nz - 1 - cz : cz;
+        // cell index and its increments: the walk advances with selects, not
+        // branches. It ends at the first boundary at or beyond t_out, which is
+        // at most the grid exit; where rounding lets it cross the grid's face
+        // first, the next index (wrapped into another row, or clamped into the
+        // table) names a cell off the segment, whose targets are tested
+        // exactly and cannot change the smallest hit index.
         const int dcx = sx, dcy = sy * g.nx, dcz = sz * g.nx * g.ny;
+        const int c_max = g.n_cells - 1;
         int c = (cz * g.ny + cy) * g.nx + cx;
-        float4 lo = cells[2 * c], hi = cells[2 * c + 1];
-        // slab parameters of a cell box as one FMA per plane: (b - p) / d =
-        // b (1/d) - p (1/d); the rounding of p (1/d) moves a plane by about
-        // |p| 2^-24 along its axis, far inside the boxes' 1 cm margin
-        const float px_ = pos.x * ix_, py_ = pos.y * iy_, pz_ = pos.z * iz_;
         const int max_visits = g.nx + g.ny + g.nz;
         for (int v = 0; v < max_visits; ++v) {
-            // next cell first, so that its record loads while this one is tested
+            // this cell's record; the walk's next step is computed while it loads
+            const float4 lo = cells[2 * c], hi = cells[2 * c + 1];
             const float tn = fminf(fminf(tmx, tmy), tmz);
             const bool ax = tn == tmx;
             const bool ay = !ax && tn == tmy;
             const bool az = !ax && !ay;
-            const int rem = ax ? rx : (ay ? ry : rz);
-            const bool last = tn >= t_out || rem == 0;
-            const int cn = last ? c : c + (ax ? dcx : (ay ? dcy : dcz));
-            const float4 lo_n = cells[2 * cn], hi_n = cells[2 * cn + 1];
-            // this cell: slab test of the segment [0, step] against the box of
-            // its targets, then the targets themselves
-            const uint32_t cnt = float_bits(hi.w);
+            const bool last = tn >= t_out;
+            const int cn = min(max(c + (ax ? dcx : (ay ? dcy : dcz)), 0), c_max);
             if (COUNT) w.cells += 1;
-            if (cnt != 0) {
-                const float ax0 = fmaf(lo.x, ix_, -px_), ax1 = fmaf(hi.x, ix_, -px_);
-                const float ay0 = fmaf(lo.y, iy_, -py_), ay1 = fmaf(hi.y, iy_, -py_);
-                const float az0 = fmaf(lo.z, iz_, -pz_), az1 = fmaf(hi.z, iz_, -pz_);
-                const float b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)),
-                                         fmaxf(fminf(az0, az1), 0.0f));
-                const float b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)),
-                                          fminf(fmaxf(az0, az1), step));
-                if (b_in <= b_out) {
-                    const uint32_t b = float_bits(lo.w);
-                    if (COUNT) w.tests += cnt;
-                    for (uint32_t k = b; k < b + cnt; ++k) {
-                        const int t = items[k];
-                        float d;
-                        if (sphere_hit(tgt[t], pos, dir, exc, step, d) && (found < 0 || t < found)) {
-                            found = t;
-                            dt = d;
-                        }
+            // slab test of the segment [0, step] against the box of this
+            // cell's targets (an empty cell holds a box at +inf, which fails
+            // it), then the targets themselves
+            const float ax0 = fmaf(lo.x, ix_, -px_), ax1 = fmaf(hi.x, ix_, -px_);
+            const float ay0 = fmaf(lo.y, iy_, -py_), ay1 = fmaf(hi.y, iy_, -py_);
+            const float az0 = fmaf(lo.z, iz_, -pz_), az1 = fmaf(hi.z, iz_, -pz_);
+            const float b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)),
+                                     fmaxf(fminf(az0, az1), 0.0f));
+            const float b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)),
+                                      fminf(fmaxf(az0, az1), step));
+            if (b_in <= b_out) {
+                const uint32_t b = float_bits(lo.w), cnt = float_bits(hi.w);
+                if (COUNT) w.tests += cnt;
+                for (uint32_t k = b; k < b + cnt; ++k) {
+                    const int t = items[k];
+                    float d;
+                    if (sphere_hit(tgt[t], pos, dir, exc, step, d) && (found < 0 || t < found)) {
+                        found = t;
+                        dt = d;
                     }
                 }
             }
             if (last) break;
             c = cn;
-            lo = lo_n;
-            hi = hi_n;
-            rx -= ax;
-            ry -= ay;
-            rz -= az;
             tmx += ax ? tdx : 0.0f;
             tmy += ay ? tdy : 0.0f;
             tmz += az ? tdz : 0.0f;

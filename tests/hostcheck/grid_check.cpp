@@ -5,7 +5,11 @@
 // under AddressSanitizer, so any out-of-bounds table access aborts the run.
 //   usage: grid_check <scene> <seed> <n_rays> <cell> [cell_z]
 //          (cell <= 0: auto; cell_z 0: cubic, < 0: columns, > 0: z edge)
-//   prints: rays hits mismatches nx ny nz n_items
+//   prints: rays hits mismatches nx ny nz n_items far_fat
+//   (far_fat: brute-force hits of a direction longer than unit whose hit point
+//   lies outside the sphere by more than the grid's margin; not mismatches)
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -60,13 +64,13 @@ int main(int argc, char** argv) {
     std::vector<uint16_t> items(t.items.begin(), t.items.begin() + std::max(t.g.n_items, 1));
     GridSearch grid{tgt.data(), cells.data(), items.data(), t.g};
     BruteSearch brute{tgt.data(), n};
-    long hits = 0, mism = 0;
+    long hits = 0, mism = 0, n_fat = 0;
     const float span = scene == 1 ? 30.0f : 300.0f;
     for (long r = 0; r < n_rays; ++r) {
         f3 pos, dir;
-        if (r % 2 == 0 && n > 0) {  // aim near a random target
+        if (r % 2 == 0 && n > 0) {  // aim near a random target, every 13th from afar
             const int k = (int)(U(rng) * n) % n;
-            const float d = 1.0f + 60.0f * U(rng);
+            const float d = r % 13 == 4 ? 2000.0f + 3000.0f * U(rng) : 1.0f + 60.0f * U(rng);
             f3 a = make3(N(rng), N(rng), N(rng));
             const float an = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
             dir = make3(a.x / an, a.y / an, a.z / an);
@@ -80,17 +84,36 @@ int main(int argc, char** argv) {
             const float an = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z) + 1e-30f;
             dir = make3(a.x / an, a.y / an, a.z / an);
         }
-        const float step = -logf(1.0f - U(rng)) * 40.0f;
+        float step = -logf(1.0f - U(rng)) * 40.0f;
+        if (r % 13 == 4) step += 5000.0f;  // long enough to reach the far target
         float d1 = 0.0f, d2 = 0.0f;
         const int f1 = brute(pos, dir, step, d1);
         const int f2 = grid(pos, dir, step, d2);
         if (f1 >= 0) ++hits;
         if (f1 != f2 || (f1 >= 0 && d1 != d2)) {
-            if (mism < 5)
-                printf("mismatch ray %ld: brute %d (%g) grid %d (%g)\n", r, f1, d1, f2, d2);
-            ++mism;
+            // The reference's discriminant keeps a^2 (|dir|^2 - 1): for a
+            // direction longer than unit its hit point lies sqrt(r^2 + e d^2)
+            // from the centre, outside the sphere. A brute-force hit whose
+            // point lies outside the sphere by more than the binning margin is
+            // such a hit (rays from afar), which the grid need not find.
+            bool fat = false;
+            if (f1 >= 0 && (f2 < 0 || f2 > f1)) {
+                const double hx = pos.x + (double)d1 * dir.x - xyz[3 * f1];
+                const double hy = pos.y + (double)d1 * dir.y - xyz[3 * f1 + 1];
+                const double hz = pos.z + (double)d1 * dir.z - xyz[3 * f1 + 2];
+                const double m = 1e-3 * std::max(t.g.h, t.g.hz) + 1e-3;
+                fat = std::sqrt(hx * hx + hy * hy + hz * hz) > rad[f1] + m;
+            }
+            if (fat) {
+                ++n_fat;
+            } else {
+                if (mism < 5)
+                    printf("mismatch ray %ld: brute %d (%g) grid %d (%g)\n", r, f1, d1, f2, d2);
+                ++mism;
+            }
         }
     }
-    printf("%ld %ld %ld %d %d %d %d\n", n_rays, hits, mism, t.g.nx, t.g.ny, t.g.nz, t.g.n_items);
+    printf("%ld %ld %ld %d %d %d %d %ld\n", n_rays, hits, mism, t.g.nx, t.g.ny, t.g.nz, t.g.n_items,
+           n_fat);
     return 0;
 }

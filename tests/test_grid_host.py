@@ -1,6 +1,13 @@
 """The target grid checked on the CPU: the device search code (pp_search.h,
 compiled for the host) under AddressSanitizer must agree with the brute-force
-first-in-index search on every segment, for several scenes and cell sizes."""
+first-in-index search on every segment, for several scenes and cell sizes.
+The exception, counted separately: the reference's discriminant keeps
+a^2 (|dir|^2 - 1), so a Float32 direction slightly longer than unit "hits" a
+sphere from far away at a point up to sqrt(r^2 + e d^2) from its centre; the
+grid, binned with a margin of centimetres, does not report such hits from
+kilometres away (2-5 km rays in this check). At those distances the
+reference's own Float32 discriminant a^2 - (|dp|^2 - r^2) carries rounding
+errors of ulp(a^2), square metres, far above r^2."""
 import os
 import subprocess
 import tempfile
@@ -34,10 +41,12 @@ def test_grid_equals_brute_force(grid_check, scene, cell, cell_z):
                        capture_output=True, text=True,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
-    rays, hits, mism, nx, ny, nz, n_items = map(int, r.stdout.split()[-7:])
+    rays, hits, mism, nx, ny, nz, n_items, fat = map(int, r.stdout.split()[-8:])
     print(f"scene {scene} cell {cell}/{cell_z}: grid {nx}x{ny}x{nz}, {n_items} entries, "
-          f"{hits} hits")
+          f"{hits} hits, {fat} far hits outside their sphere")
     assert hits > 1000
     assert mism == 0
+    # only rays from 2-5 km (1 in 26) can carry such hits
+    assert fat < rays / 26 * 0.2
     if cell_z < 0:
         assert nz == 1
