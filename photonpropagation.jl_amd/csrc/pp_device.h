@@ -218,6 +218,17 @@ __device__ __forceinline__ f3 update_direction(f3 dir, float ct, float u_phi) {
 }
 
 // ---------------------------------------------------------------------------
+// Step length   src/photon_prop_cuda.jl:488-490
+// step = -log(eta) sca_len = -log2(eta) (ln2 sca_len): the hardware log2
+// (eta >= 2^-24 is a normal number) times a per-photon constant, instead of
+// the library logf's denormal rescale and extended-precision ln2 product.
+// ---------------------------------------------------------------------------
+constexpr float kLn2 = 0.693147180559945309f;
+__device__ __forceinline__ float step_length(float eta, float ln2_sca_len) {
+    return -__builtin_amdgcn_logf(eta) * ln2_sca_len;
+}
+
+// ---------------------------------------------------------------------------
 // Ray-sphere test: _intersect_nsphere   src/photon_prop_cuda.jl:309-351
 // Near root d = -a - sqrt(b), a = dir.dp, b = a^2 - (|dp|^2 - r^2); hit iff
 // b >= 0 and 0 < d < step. b is evaluated as

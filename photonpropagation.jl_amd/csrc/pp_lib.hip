@@ -283,12 +283,12 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
     const PhotonInit st = init_photon(p, src, spec, s);
     f3 pos = st.pos, dir = st.dir;
     float time = st.time, dist = 0.0f;
-    const float sca_len = st.mv.sca_len, inv_c = st.mv.inv_c_grp;
+    const float ln2_sca = st.mv.sca_len * pp::kLn2, inv_c = st.mv.inv_c_grp;
     bool hit = false;
     int32_t module_ix = 0, n_at = 0;
     for (int32_t step = 1; step <= p.n_steps; ++step) {
         const uint4 r = s.block((uint32_t)step);
-        const float step_size = -logf(pp::u01_open0(r.x)) * sca_len;
+        const float step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
         // first target in index order whose near root lies inside the step
         float dt = 0.0f;
         const int found = search.template find<COUNT>(pos, dir, step_size, dt, work);
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         break;
     }
     case PP_FN_STEP_LEN:
-        y[0] = -logf(x[0]) * x[1];
+        y[0] = pp::step_length(x[0], x[1] * pp::kLn2);
         break;
     case PP_FN_CASCADIA: {
         pp::MediumValues mv = pp::medium_values(e.med, x[0]);

@@ -60,18 +60,15 @@ struct BruteSearch {
     }
 };
 
-__host__ __device__ __forceinline__ float nonzero(float x) {
-    return copysignf(fmaxf(fabsf(x), 1e-20f), x);
-}
-
-// Reciprocal for the grid walk: the hardware estimate on the device (the walk
-// only needs its boundary parameters to be consistent, not correctly
-// rounded); a plain division when the same code is compiled for the host.
-__host__ __device__ __forceinline__ float fast_rcp(float x) {
+// Reciprocal for the grid walk, clamped to +-1e20 (1/+-0 = +-inf becomes
+// +-1e20): the hardware estimate on the device (the walk only needs its
+// boundary parameters to be consistent, not correctly rounded); a plain
+// division when the same code is compiled for the host.
+__host__ __device__ __forceinline__ float rcp_clamped(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_rcpf(x);
+    return __builtin_amdgcn_fmed3f(__builtin_amdgcn_rcpf(x), -1e20f, 1e20f);
 #else
-    return 1.0f / x;
+    return fminf(fmaxf(1.0f / x, -1e20f), 1e20f);
 #endif
 }
 
@@ -134,10 +131,10 @@ struct GridSearch {
     template <bool COUNT>
     __host__ __device__ __forceinline__ int find(f3 pos, f3 dir, float step, float& dt,
                                                  Work& w) const {
-        // direction components pushed away from zero; the walk's step signs
-        // below come from these same values so that they agree with ix_..iz_
-        const float ix_ = fast_rcp(nonzero(dir.x)), iy_ = fast_rcp(nonzero(dir.y)),
-                    iz_ = fast_rcp(nonzero(dir.z));
+        // reciprocal direction components, finite (|1/d| <= 1e20, the sign
+        // of a zero component kept); the walk's step signs below come from
+        // these same values so that they agree with ix_..iz_
+        const float ix_ = rcp_clamped(dir.x), iy_ = rcp_clamped(dir.y), iz_ = rcp_clamped(dir.z);
         // plane parameters as one FMA per plane: (b - p) / d = b (1/d) - p (1/d);
         // the rounding of p (1/d) moves a plane by about |p| 2^-24 along its
         // axis, far inside the binning margin and the boxes' 1 cm margin
@@ -151,7 +148,6 @@ struct GridSearch {
         const float t_out = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), step));
         int found = -1;
         if (!(t_in <= t_out)) return found;
-        const float exc = dir_excess(dir);
         // entry cell
         const float px = pos.x + dir.x * t_in, py = pos.y + dir.y * t_in, pz = pos.z + dir.z * t_in;
         const int cx = clampi((int)floorf((px - g.ox) * g.inv_h), g.nx - 1);
@@ -195,6 +191,7 @@ struct GridSearch {
                                       fminf(fmaxf(az0, az1), step));
             if (b_in <= b_out) {
                 const uint32_t b = float_bits(lo.w), cnt = float_bits(hi.w);
+                const float exc = dir_excess(dir);  // rarely needed: not kept across the walk
                 if (COUNT) w.tests += cnt;
                 for (uint32_t k = b; k < b + cnt; ++k) {
                     const int t = items[k];
