@@ -323,7 +323,7 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* model, const float* wl_grid,
  *   block size        64, 128, 256, 512, 1024
  *   photons per lane  per work chunk: 1, 2, 4, 8, 16 (chunk_photons = 64 x),
  *                     or the automatic chunk
- *   grid cell edge    automatic, and 0.6x, 1.5x, 2.5x, 4x the automatic edge
+ *   grid cell edge    automatic, and 0.6x, 0.8x, 0.9x, 1.1x, 1.25x the automatic edge
  *                     (grid search only)
  *   grid cell shape   cubic, or columns over the whole height (grid only)
  * with the batch mode of the context (PP_BATCH_AUTO: persistent). Each

@@ -40,12 +40,13 @@ inline float u32_as_float(uint32_t u) {
 // float rounding of the DDA walk). Each cell lists its targets in ascending
 // index order and keeps the union of their boxes, grown by a further
 // 1 cm + 1e-4 max(h, hz), so that a float slab test of a segment against it can
-// only err towards testing; an empty cell keeps a box at +inf. cell_req <= 0 picks 1.5x the edge of one cubic
-// cell per target (about 3.4 targets per cell); cell_z_req = 0 makes the cells
-// cubic, < 0 one layer over the whole height (columns: strings are vertical),
-// > 0 sets the z edge. The edges grow by 1.25x until the tables fit `budget`
-// bytes of LDS together with the targets and n_spec spectrum knots. Returns
-// false if nothing fits.
+// only err towards testing; an empty cell keeps a box at +inf. cell_req > 0
+// sets the xy edge; <= 0 picks, among 1-2.5x the edge of one cubic cell per
+// target, the edge of least modelled walk cost (below). cell_z_req = 0 makes
+// the cells cubic, < 0 one layer over the whole height (columns: strings are
+// vertical), > 0 sets the z edge. A set edge grows by 1.25x until the tables
+// fit `budget` bytes of LDS together with the targets and n_spec spectrum
+// knots. Returns false if nothing fits.
 inline bool build_grid_tables(const float* xyz, const float* radius, int n, float cell_req,
                               float cell_z_req, int n_spec, size_t budget, GridTables& out) {
     if (n <= 0 || n > 65535) return false;
@@ -57,11 +58,9 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         }
     float ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = std::max(hi[a] - lo[a], 1e-3f);
-    float h = cell_req;
-    if (!(h > 0.0f)) h = 1.5f * std::cbrt(ext[0] * ext[1] * ext[2] / (float)n);
-    h = std::max(h, 1e-3f);
-    float hz0 = cell_z_req > 0.0f ? std::max(cell_z_req, 1e-3f) : 0.0f;
-    for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f, hz0 *= 1.25f) {
+    // one grid of xy edge h and z edge hz0 (cubic if cell_z_req == 0, columns
+    // if < 0); false if its tables exceed the budget
+    auto try_build = [&](float h, float hz0, GridTables& out) -> bool {
         // column cells: one layer whose edge covers the extent with margins
         float hz = cell_z_req == 0.0f ? h : hz0;
         float m = 1e-3f * std::max(h, hz) + 1e-3f;
@@ -73,11 +72,13 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         float o[3];
         int dim[3];
         for (int a = 0; a < 3; ++a) {
-            o[a] = lo[a] - 2.0f * m;
+            // whole cells over the extent and margins, centred on the targets
+            // (targets on a regular pitch equal to the edge sit mid-cell)
             dim[a] = std::max(1, (int)std::ceil((ext[a] + 4.0f * m) / edge[a]));
+            o[a] = lo[a] - 2.0f * m - 0.5f * ((float)dim[a] * edge[a] - (ext[a] + 4.0f * m));
         }
         const int64_t n_cells = (int64_t)dim[0] * dim[1] * dim[2];
-        if (n_cells * 32 > (int64_t)budget) continue;
+        if (n_cells * 32 > (int64_t)budget) return false;
         auto range = [&](int i, int a, int& c0, int& c1) {
             const float p = xyz[3 * i + a], r = radius[i];
             c0 = std::max(0, (int)std::floor((p - r - m - o[a]) / edge[a]));
@@ -97,7 +98,7 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
                         ++n_items;
                     }
         }
-        if (lds_bytes(n, (int)n_cells, (int)n_items, n_spec, true) > budget) continue;
+        if (lds_bytes(n, (int)n_cells, (int)n_items, n_spec, true) > budget) return false;
         std::vector<uint32_t> start((size_t)n_cells + 1, 0);
         for (int64_t k = 0; k < n_cells; ++k)
             start[(size_t)k + 1] = start[(size_t)k] + count[(size_t)k];
@@ -145,7 +146,52 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         g.n_cells = (int)n_cells;
         g.n_items = (int)n_items;
         return true;
+    };
+    const float hz_req = cell_z_req > 0.0f ? std::max(cell_z_req, 1e-3f) : 0.0f;
+    if (cell_req > 0.0f) {
+        float h = std::max(cell_req, 1e-3f), hz0 = hz_req;
+        for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f, hz0 *= 1.25f)
+            if (try_build(h, hz0, out)) return true;
+        return false;
     }
+    // automatic edge: the candidate that minimises a walk cost model. A segment
+    // of unit length with a uniform position and direction crosses on average
+    // (1/h + 1/h + 1/hz) / 2 cell faces and meets a box of surface S with
+    // probability S / (4 V) per unit of its length in the grid volume V
+    // (Cauchy). Cost per unit length = 42 visits + 80 member tests, the
+    // instructions of one cell visit and of one sphere test at low lane
+    // occupancy. Bigger cells are visited less, but once a cell holds
+    // targets far apart (two strings) its member box grows and the tests
+    // explode; the model sees both.
+    const float base = std::cbrt(ext[0] * ext[1] * ext[2] / (float)n);
+    bool found = false;
+    double best = INFINITY;
+    for (float f : {1.0f, 1.25f, 1.5f, 1.75f, 2.0f, 2.25f, 2.5f}) {
+        const float h = std::max(f * base, 1e-3f);
+        GridTables t;
+        if (!try_build(h, hz_req, t)) continue;
+        const GridDesc& g = t.g;
+        const double vol = (double)g.nx * g.h * g.ny * g.h * g.nz * g.hz;
+        double surf = 0.0;
+        for (int c = 0; c < g.n_cells; ++c) {
+            const float4 a = t.cells[2 * (size_t)c], b = t.cells[2 * (size_t)c + 1];
+            uint32_t cnt;
+            std::memcpy(&cnt, &b.w, 4);
+            if (cnt == 0) continue;
+            const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+            surf += cnt * 2.0 * (dx * dy + dy * dz + dx * dz);
+        }
+        const double cost = 42.0 * 0.5 * (2.0 / g.h + 1.0 / g.hz) + 80.0 * surf / (4.0 * vol);
+        if (cost < best) {
+            best = cost;
+            out = std::move(t);
+            found = true;
+        }
+    }
+    if (found) return true;
+    float h = 2.5f * base, hz0 = hz_req;
+    for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f, hz0 *= 1.25f)
+        if (try_build(h, hz0, out)) return true;
     return false;
 }
 

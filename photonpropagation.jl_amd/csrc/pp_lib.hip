@@ -1272,7 +1272,7 @@ int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t s
     const float h_auto = grid ? ctx->grid.h : 0.0f;
     std::vector<float> cells = {0.0f};
     if (grid)
-        for (float f : {0.6f, 1.5f, 2.5f, 4.0f}) cells.push_back(f * h_auto);
+        for (float f : {0.6f, 0.8f, 0.9f, 1.1f, 1.25f}) cells.push_back(f * h_auto);
     const int blocks[] = {64, 128, 256, 512, 1024};
     const int chunks[] = {0, 64, 128, 256, 512, 1024};
     const float shapes[] = {0.0f, -1.0f};  // cubic cells, columns
