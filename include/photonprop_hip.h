@@ -325,7 +325,8 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* model, const float* wl_grid,
  *                     or the automatic chunk
  *   grid cell edge    automatic, and 0.6x, 0.8x, 0.9x, 1.1x, 1.25x the automatic edge
  *                     (grid search only)
- *   grid cell shape   cubic, or columns over the whole height (grid only)
+ *   grid cell shape   cubic, twice as tall as wide, or columns over the whole
+ *                     height (grid only)
  * with the batch mode of the context (PP_BATCH_AUTO: persistent). Each
  * candidate runs once untimed and twice timed (HIP events); the faster run
  * counts. The fastest configuration is written to *best (if not NULL) and

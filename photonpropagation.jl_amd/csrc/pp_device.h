@@ -243,12 +243,16 @@ __host__ __device__ __forceinline__ float dir_excess(f3 dir) {
     return (float)::fma(x, x, ::fma(y, y, ::fma(z, z, -1.0)));
 }
 
+// Every product and sum is spelled out (fmaf, or a plain product that cannot
+// be fused), so that the result does not depend on where the compiler inlines
+// it: the grid and the brute-force search give identical distances.
 __host__ __device__ __forceinline__ bool sphere_hit(float4 t, f3 pos, f3 dir, float excess,
                                                     float step, float& d) {
     const float dx = pos.x - t.x, dy = pos.y - t.y, dz = pos.z - t.z;
-    const float a = dir.x * dx + dir.y * dy + dir.z * dz;
-    const float qx = dx - a * dir.x, qy = dy - a * dir.y, qz = dz - a * dir.z;
-    const float b = (a * excess) * a + (t.w - (qx * qx + qy * qy + qz * qz));  // t.w = r^2
+    const float a = fmaf(dir.z, dz, fmaf(dir.y, dy, dir.x * dx));
+    const float qx = fmaf(-a, dir.x, dx), qy = fmaf(-a, dir.y, dy), qz = fmaf(-a, dir.z, dz);
+    const float q2 = fmaf(qz, qz, fmaf(qy, qy, qx * qx));
+    const float b = fmaf(a * excess, a, t.w - q2);  // t.w = r^2
     d = (b >= 0.0f) ? (-a - sqrt_hw(b)) : -1.0f;
     return (d > 0.0f) & (d < step);
 }
@@ -265,7 +269,7 @@ __host__ __device__ __forceinline__ bool plane_hit(uint32_t kind, float4 t, floa
     if (dir.z == 0.0f) return false;
     d = (t.z - pos.z) / dir.z;
     if (d < 0.0f || d > step) return false;
-    const float x = (pos.x + dir.x * d) - t.x, y = (pos.y + dir.y * d) - t.y;
+    const float x = fmaf(dir.x, d, pos.x) - t.x, y = fmaf(dir.y, d, pos.y) - t.y;
     return kind == PP_SHAPE_RECTANGLE ? (fabsf(x) < e.y && fabsf(y) < e.z)
                                       : (x * x + y * y < t.w);
 }

@@ -145,6 +145,9 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         g.nz = dim[2];
         g.n_cells = (int)n_cells;
         g.n_items = (int)n_items;
+        float r_max = 0.0f;
+        for (int i = 0; i < n; ++i) r_max = std::max(r_max, radius[i]);
+        g.zpad = r_max + mb;
         return true;
     };
     const float hz_req = cell_z_req > 0.0f ? std::max(cell_z_req, 1e-3f) : 0.0f;

@@ -1275,11 +1275,13 @@ int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t s
         for (float f : {0.6f, 0.8f, 0.9f, 1.1f, 1.25f}) cells.push_back(f * h_auto);
     const int blocks[] = {64, 128, 256, 512, 1024};
     const int chunks[] = {0, 64, 128, 256, 512, 1024};
-    const float shapes[] = {0.0f, -1.0f};  // cubic cells, columns
+    // cell shapes: cubic, columns, and cells twice as tall as wide (set when tried)
+    float shapes[] = {0.0f, -1.0f, 0.0f};
     for (int pass = 0; pass < 2; ++pass) {
         for (int knob = 0; knob < 4; ++knob) {
             if (knob >= 2 && !grid) continue;
-            const size_t n_cand = knob == 0 ? 5 : knob == 1 ? 6 : knob == 2 ? cells.size() : 2;
+            const size_t n_cand = knob == 0 ? 5 : knob == 1 ? 6 : knob == 2 ? cells.size() : 3;
+            if (knob == 3) shapes[2] = 2.0f * (cur.grid_cell > 0.0f ? cur.grid_cell : h_auto);
             for (size_t k = 0; k < n_cand; ++k) {
                 pp_config c = cur;
                 if (knob == 0) c.block_size = blocks[k];

@@ -29,6 +29,7 @@ struct GridDesc {
     float hz, inv_hz;  // cell edge along z (strings are vertical: cells may be columns)
     int nx, ny, nz;
     int n_cells, n_items;
+    float zpad;        // largest target radius plus the member boxes' margin
 };
 
 // Work done by one search (instrumented builds only).
@@ -192,11 +193,19 @@ struct GridSearch {
             if (b_in <= b_out) {
                 const uint32_t b = float_bits(lo.w), cnt = float_bits(hi.w);
                 const float exc = dir_excess(dir);  // rarely needed: not kept across the walk
-                if (COUNT) w.tests += cnt;
+                // a target is hit inside the box, so within its radius of the
+                // segment's z range there: targets farther in z are skipped
+                // (columns hold a whole string; a ray crossing it meets one or
+                // two of its modules)
+                const float za = fmaf(dir.z, b_in, pos.z), zb = fmaf(dir.z, b_out, pos.z);
+                const float zlo = fminf(za, zb) - g.zpad, zhi = fmaxf(za, zb) + g.zpad;
                 for (uint32_t k = b; k < b + cnt; ++k) {
                     const int t = items[k];
+                    const float4 c = tgt[t];
+                    if (c.z < zlo || c.z > zhi) continue;
+                    if (COUNT) w.tests += 1;
                     float d;
-                    if (sphere_hit(tgt[t], pos, dir, exc, step, d) && (found < 0 || t < found)) {
+                    if (sphere_hit(c, pos, dir, exc, step, d) && (found < 0 || t < found)) {
                         found = t;
                         dt = d;
                     }
