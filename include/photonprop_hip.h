@@ -162,14 +162,17 @@ typedef struct pp_config {
     int32_t block_size;         /* threads per block, multiple of 64; 0 = 256 */
     int32_t waves_per_simd;     /* grid sizing: blocks = CUs*4*wps*64/block; 0 = auto */
     int32_t search;             /* enum pp_search_kind                         */
-    float grid_cell;            /* grid cell edge (m); 0 = auto                */
+    float grid_cell;            /* grid cell edge (m); 0 = auto: the edge (and,
+                                   with grid_cell_z = 0, the shape) of least
+                                   modelled walk cost                          */
     int32_t deterministic_order;/* pp_copy_hits sorts by (source, photon id)   */
     int32_t count_work;         /* instrumented kernel: counts steps, sphere
                                    tests and grid-cell visits (pp_work_counts) */
     int32_t batch_mode;         /* enum pp_batch_mode                          */
     int32_t chunk_photons;      /* photons per work chunk (multiple of 64);
                                    0 = auto                                    */
-    float grid_cell_z;          /* grid cell edge along z (m): 0 = cubic cells,
+    float grid_cell_z;          /* grid cell edge along z (m): 0 = cubic cells
+                                   (automatic shape if grid_cell is 0 too),
                                    < 0 = one layer over the targets' height
                                    (columns, suited to vertical strings)       */
     int32_t reserved[7];

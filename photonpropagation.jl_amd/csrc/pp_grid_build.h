@@ -42,9 +42,10 @@ inline float u32_as_float(uint32_t u) {
 // 1 cm + 1e-4 max(h, hz), so that a float slab test of a segment against it can
 // only err towards testing; an empty cell keeps a box at +inf. cell_req > 0
 // sets the xy edge; <= 0 picks, among 1-2.5x the edge of one cubic cell per
-// target, the edge of least modelled walk cost (below). cell_z_req = 0 makes
-// the cells cubic, < 0 one layer over the whole height (columns: strings are
-// vertical), > 0 sets the z edge. A set edge grows by 1.25x until the tables
+// target, the edge of least modelled walk cost (below). cell_z_req < 0 makes
+// one layer over the whole height (columns: strings are vertical), > 0 sets
+// the z edge, 0 makes the cells cubic - or, with cell_req <= 0 as well, picks
+// cubic cells or columns by the same model. A set edge grows by 1.25x until the tables
 // fit `budget` bytes of LDS together with the targets and n_spec spectrum
 // knots. Returns false if nothing fits.
 inline bool build_grid_tables(const float* xyz, const float* radius, int n, float cell_req,
@@ -58,13 +59,14 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         }
     float ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = std::max(hi[a] - lo[a], 1e-3f);
-    // one grid of xy edge h and z edge hz0 (cubic if cell_z_req == 0, columns
-    // if < 0); false if its tables exceed the budget
-    auto try_build = [&](float h, float hz0, GridTables& out) -> bool {
+    // one grid of xy edge h; z edge h (zmode 0, cubic), hz0 (zmode > 0) or
+    // the whole height (zmode < 0, columns); false if its tables exceed the
+    // budget
+    auto try_build = [&](float h, int zmode, float hz0, GridTables& out) -> bool {
         // column cells: one layer whose edge covers the extent with margins
-        float hz = cell_z_req == 0.0f ? h : hz0;
+        float hz = zmode == 0 ? h : hz0;
         float m = 1e-3f * std::max(h, hz) + 1e-3f;
-        if (cell_z_req < 0.0f) {
+        if (zmode < 0) {
             hz = ext[2] + 4.0f * (1e-3f * std::max(h, ext[2]) + 1e-3f) + 1e-2f;
             m = 1e-3f * std::max(h, hz) + 1e-3f;
         }
@@ -151,50 +153,60 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         return true;
     };
     const float hz_req = cell_z_req > 0.0f ? std::max(cell_z_req, 1e-3f) : 0.0f;
+    const int zmode_req = cell_z_req > 0.0f ? 1 : cell_z_req < 0.0f ? -1 : 0;
     if (cell_req > 0.0f) {
         float h = std::max(cell_req, 1e-3f), hz0 = hz_req;
         for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f, hz0 *= 1.25f)
-            if (try_build(h, hz0, out)) return true;
+            if (try_build(h, zmode_req, hz0, out)) return true;
         return false;
     }
-    // automatic edge: the candidate that minimises a walk cost model. A segment
-    // of unit length with a uniform position and direction crosses on average
-    // (1/h + 1/h + 1/hz) / 2 cell faces and meets a box of surface S with
-    // probability S / (4 V) per unit of its length in the grid volume V
-    // (Cauchy). Cost per unit length = 42 visits + 80 member tests, the
-    // instructions of one cell visit and of one sphere test at low lane
-    // occupancy. Bigger cells are visited less, but once a cell holds
-    // targets far apart (two strings) its member box grows and the tests
-    // explode; the model sees both.
+    // Automatic edge (and, when cell_z_req is 0 too, shape): the candidate of
+    // least modelled walk cost per unit length. A segment of unit length with
+    // a uniform position and direction crosses (1/h + 1/h + 1/hz) / 2 cell
+    // faces on average and meets a box of surface S with probability S / (4 V)
+    // (Cauchy's formula, V the grid volume). A visit costs ~42 instructions
+    // (~37 in the 2-D walk of one layer); a box that is met costs 8 per member
+    // (the z-range filter) plus 40 per member left to test, a fraction
+    // (2 zpad + w) / height of them for a box of width w. Bigger cells are
+    // visited less, but once a cell holds targets far apart (two strings) its
+    // member box spans the gap and is met far more often; the model sees both.
     const float base = std::cbrt(ext[0] * ext[1] * ext[2] / (float)n);
     bool found = false;
     double best = INFINITY;
-    for (float f : {1.0f, 1.25f, 1.5f, 1.75f, 2.0f, 2.25f, 2.5f}) {
-        const float h = std::max(f * base, 1e-3f);
-        GridTables t;
-        if (!try_build(h, hz_req, t)) continue;
-        const GridDesc& g = t.g;
-        const double vol = (double)g.nx * g.h * g.ny * g.h * g.nz * g.hz;
-        double surf = 0.0;
-        for (int c = 0; c < g.n_cells; ++c) {
-            const float4 a = t.cells[2 * (size_t)c], b = t.cells[2 * (size_t)c + 1];
-            uint32_t cnt;
-            std::memcpy(&cnt, &b.w, 4);
-            if (cnt == 0) continue;
-            const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
-            surf += cnt * 2.0 * (dx * dy + dy * dz + dx * dz);
-        }
-        const double cost = 42.0 * 0.5 * (2.0 / g.h + 1.0 / g.hz) + 80.0 * surf / (4.0 * vol);
-        if (cost < best) {
-            best = cost;
-            out = std::move(t);
-            found = true;
+    const int zmodes[2] = {zmode_req, -1};
+    const int n_modes = cell_z_req == 0.0f ? 2 : 1;  // fully automatic: cubic or columns
+    for (int mi = 0; mi < n_modes; ++mi) {
+        for (float f : {1.0f, 1.25f, 1.5f, 1.75f, 2.0f, 2.25f, 2.5f}) {
+            const float h = std::max(f * base, 1e-3f);
+            GridTables t;
+            if (!try_build(h, zmodes[mi], hz_req, t)) continue;
+            const GridDesc& g = t.g;
+            const double vol = (double)g.nx * g.h * g.ny * g.h * g.nz * g.hz;
+            double met = 0.0;
+            for (int c = 0; c < g.n_cells; ++c) {
+                const float4 a = t.cells[2 * (size_t)c], b = t.cells[2 * (size_t)c + 1];
+                uint32_t cnt;
+                std::memcpy(&cnt, &b.w, 4);
+                if (cnt == 0) continue;
+                const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+                const double surf = 2.0 * (dx * dy + dy * dz + dx * dz);
+                const double frac = std::min(1.0, (2.0 * g.zpad + std::min(dx, dy)) / dz);
+                met += surf * cnt * (8.0 + 40.0 * frac);
+            }
+            const double per_visit = g.nz == 1 ? 37.0 : 42.0;
+            const double cost = per_visit * 0.5 * (2.0 / g.h + (g.nz == 1 ? 0.0 : 1.0 / g.hz)) +
+                                met / (4.0 * vol);
+            if (cost < best) {
+                best = cost;
+                out = std::move(t);
+                found = true;
+            }
         }
     }
     if (found) return true;
     float h = 2.5f * base, hz0 = hz_req;
     for (int attempt = 0; attempt < 200; ++attempt, h *= 1.25f, hz0 *= 1.25f)
-        if (try_build(h, hz0, out)) return true;
+        if (try_build(h, zmode_req, hz0, out)) return true;
     return false;
 }
 

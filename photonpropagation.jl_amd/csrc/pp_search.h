@@ -132,6 +132,13 @@ struct GridSearch {
     template <bool COUNT>
     __host__ __device__ __forceinline__ int find(f3 pos, f3 dir, float step, float& dt,
                                                  Work& w) const {
+        // one layer of cells (columns over the whole height): a 2-D walk
+        return g.nz == 1 ? walk<COUNT, true>(pos, dir, step, dt, w)
+                         : walk<COUNT, false>(pos, dir, step, dt, w);
+    }
+    template <bool COUNT, bool COLS>
+    __host__ __device__ __forceinline__ int walk(f3 pos, f3 dir, float step, float& dt,
+                                                 Work& w) const {
         // reciprocal direction components, finite (|1/d| <= 1e20, the sign
         // of a zero component kept); the walk's step signs below come from
         // these same values so that they agree with ix_..iz_
@@ -153,12 +160,13 @@ struct GridSearch {
         const float px = pos.x + dir.x * t_in, py = pos.y + dir.y * t_in, pz = pos.z + dir.z * t_in;
         const int cx = clampi((int)floorf((px - g.ox) * g.inv_h), g.nx - 1);
         const int cy = clampi((int)floorf((py - g.oy) * g.inv_h), g.ny - 1);
-        const int cz = clampi((int)floorf((pz - g.oz) * g.inv_hz), g.nz - 1);
+        const int cz = COLS ? 0 : clampi((int)floorf((pz - g.oz) * g.inv_hz), g.nz - 1);
         const int sx = ix_ > 0.0f ? 1 : -1, sy = iy_ > 0.0f ? 1 : -1, sz = iz_ > 0.0f ? 1 : -1;
-        // ray parameter of the next cell boundary on each axis
+        // ray parameter of the next cell boundary on each axis (in one layer
+        // the walk never crosses a z face: the clip already ends it there)
         float tmx = fmaf(g.ox + (float)(cx + (sx > 0)) * g.h, ix_, -px_);
         float tmy = fmaf(g.oy + (float)(cy + (sy > 0)) * g.h, iy_, -py_);
-        float tmz = fmaf(g.oz + (float)(cz + (sz > 0)) * g.hz, iz_, -pz_);
+        float tmz = COLS ? INFINITY : fmaf(g.oz + (float)(cz + (sz > 0)) * g.hz, iz_, -pz_);
         const float tdx = g.h * fabsf(ix_), tdy = g.h * fabsf(iy_), tdz = g.hz * fabsf(iz_);
         // cell index and its increments: the walk advances with selects, not
         // branches. It ends at the first boundary at or beyond t_out, which is
@@ -173,9 +181,9 @@ struct GridSearch {
         for (int v = 0; v < max_visits; ++v) {
             // this cell's record; the walk's next step is computed while it loads
             const float4 lo = cells[2 * c], hi = cells[2 * c + 1];
-            const float tn = fminf(fminf(tmx, tmy), tmz);
+            const float tn = COLS ? fminf(tmx, tmy) : fminf(fminf(tmx, tmy), tmz);
             const bool ax = tn == tmx;
-            const bool ay = !ax && tn == tmy;
+            const bool ay = COLS ? !ax : !ax && tn == tmy;
             const bool az = !ax && !ay;
             const bool last = tn >= t_out;
             const int cn = min(max(c + (ax ? dcx : (ay ? dcy : dcz)), 0), c_max);
@@ -215,7 +223,7 @@ struct GridSearch {
             c = cn;
             tmx += ax ? tdx : 0.0f;
             tmy += ay ? tdy : 0.0f;
-            tmz += az ? tdz : 0.0f;
+            if (!COLS) tmz += az ? tdz : 0.0f;
         }
         return found;
     }
