@@ -81,6 +81,7 @@ static void ppo_block(uint32_t word, uint64_t pid, uint32_t src, const uint32_t 
 /* exact value of every step function.                                 */
 /* ------------------------------------------------------------------ */
 #define PPO_TWO_PI_F ((float)(2.0 * 3.14159265358979323846)) /* T(2*pi) */
+#define PPO_TWO_PI_D (2.0 * 3.14159265358979323846)
 #define PPO_C_VAC_M_NS 0.299792458 /* src/photon_prop_cuda.jl:37 */
 
 /* uniform(min, max) = min + u*(max-min)   src/photon_prop_cuda.jl:48-51 */
@@ -336,7 +337,7 @@ typedef struct {
 static float ppo_normal(const ppo_stream* s, uint32_t sub) {
     uint32_t r[4];
     ppo_block(PP_CTR_SAMPLER | (sub << 24), s->pid, s->src, s->key, r);
-    double u1 = ppo_u01_open0(r[0]), phi = ppo_phi(ppo_u01(r[1]));
+    double u1 = ppo_u01_open0(r[0]), phi = PPO_TWO_PI_D * (double)ppo_u01(r[1]);
     return (float)(sqrt(-2.0 * log(u1)) * cos(phi));
 }
 
@@ -360,7 +361,7 @@ float ppo_gamma(float shape, float scale, const ppo_stream* s, uint32_t sub) {
         uint32_t r[4];
         ppo_block(PP_CTR_SAMPLER | (sub << 24) | i, s->pid, s->src, s->key, r);
         double rad = sqrt(-2.0 * log((double)ppo_u01_open0(r[0])));
-        double phi = ppo_phi(ppo_u01(r[1]));
+        double phi = PPO_TWO_PI_D * (double)ppo_u01(r[1]);  /* Box-Muller angle 2 pi u */
         double z[2] = {rad * cos(phi), rad * sin(phi)};
         double ua[2] = {ppo_u01_open0(r[2]), ppo_u01_open0(r[3])};
         for (int j = 0; j < 2; ++j) {
