@@ -83,3 +83,34 @@ def test_pmt_count_table(P, tmp_path):
     hits = np.array(t.column("hits").to_pylist())
     assert hits.shape == (3, 16) and np.all(hits >= 0) and hits.sum() > 0
     np.testing.assert_allclose(np.linalg.norm(np.array(rows["pos"]), axis=1), 15.0, rtol=1e-6)
+
+
+def test_photon_table_hdf5(P, tmp_path):
+    """The per-photon table writer (photon_tables_photons.jl:30-148): two
+    simulations appended as photons/dataset_1..2 (save_hdf!), 10 float64
+    columns stored as HDF5.jl stores a Julia matrix, residuals t - t_geo - t0
+    of photons that reached the module, the simulation's attributes."""
+    from photonprop import hdf5, tables
+    out = str(tmp_path / "photons.h5")
+    for k, (mode, cos_t) in enumerate((("extended", 0.3), ("lightsabre_muon", -0.2))):
+        name = tables.photon_table_photons(out, 1e4, 20.0, cos_t, 1.0, 100 + k, mode=mode,
+                                           n_ph_limit=5000, log=print)
+        assert name == f"dataset_{k + 1}"
+    g = hdf5.read_hdf(out)["members"]["photons"]
+    assert g["attrs"]["nsims"] == 2
+    for k in (1, 2):
+        d = g["members"][f"dataset_{k}"]
+        t = d["data"].T
+        assert t.shape[1] == 10 and 100 <= t.shape[0] <= 5000
+        cols = dict(zip(tables.PHOTON_TABLE_COLUMNS, t.T))
+        assert np.all(cols["module_id"] == 1)
+        # photons hit the 0.3 m module at the origin
+        r = np.sqrt(cols["pos_x"] ** 2 + cols["pos_y"] ** 2 + cols["pos_z"] ** 2)
+        np.testing.assert_allclose(r, 0.3, rtol=1e-4)
+        np.testing.assert_allclose(cols["dir_x"] ** 2 + cols["dir_y"] ** 2 + cols["dir_z"] ** 2,
+                                   1.0, rtol=1e-5)
+        assert np.all(cols["total_weight"] > 0) and np.all(cols["tres"] > -5.0)
+        assert np.all((cols["wavelength"] >= 300) & (cols["wavelength"] <= 800))
+        a = d["attrs"]
+        assert a["mode"] == ("extended", "lightsabre_muon")[k - 1] and a["seed"] == 99 + k
+        assert abs(a["distance"] - 20.0) < 1e-12 and "source_pos" in a
