@@ -181,10 +181,13 @@ struct GridSearch {
         for (int v = 0; v < max_visits; ++v) {
             // this cell's record; the walk's next step is computed while it loads
             const float4 lo = cells[2 * c], hi = cells[2 * c + 1];
-            const float tn = COLS ? fminf(tmx, tmy) : fminf(fminf(tmx, tmy), tmz);
-            const bool ax = tn == tmx;
-            const bool ay = COLS ? !ax : !ax && tn == tmy;
+            // axis of the nearest boundary (ties: x, then y), by compares and
+            // selects rather than fminf, whose NaN rules cost a canonicalising
+            // instruction per operand here
+            const bool ax = COLS ? tmx <= tmy : tmx <= tmy && tmx <= tmz;
+            const bool ay = COLS ? !ax : !ax && tmy <= tmz;
             const bool az = !ax && !ay;
+            const float tn = ax ? tmx : (ay ? tmy : tmz);
             const bool last = tn >= t_out;
             const int cn = min(max(c + (ax ? dcx : (ay ? dcy : dcz)), 0), c_max);
             if (COUNT) w.cells += 1;
