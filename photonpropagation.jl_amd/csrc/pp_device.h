@@ -93,29 +93,13 @@ __device__ __forceinline__ float cbrt_pos(float x) {
     return fmaf(fmaf(x, rcp_nr(y * y), -y), 1.0f / 3.0f, y);
 }
 
-// sin(2 pi u), cos(2 pi u) for u in [0, 1): exact reduction to an octant
-// (u*8 is exact), Taylor polynomials on [0, pi/4] (truncation < 2e-9), and
-// octant symmetry. ~25 instructions instead of a general sincosf.
+// sin(2 pi u), cos(2 pi u) for u in [0, 1): the hardware v_sin_f32/v_cos_f32,
+// whose argument is in revolutions (no reduction needed). Measured on gfx950
+// over all 2^24 u = k 2^-24: absolute error <= 1.25e-7 (1.05 ulp of 1), the
+// scale at which direction components are compared.
 __device__ __forceinline__ void sincos_2pi(float u, float* s, float* c) {
-    const float v = u * 8.0f;
-    const float fo = floorf(v);
-    const int o = (int)fo & 7;
-    const float f = v - fo;                    // exact
-    const float x = ((o & 1) ? 1.0f - f : f) * 0.785398163397448310f;  // [0, pi/4]
-    const float x2 = x * x;
-    float sn = fmaf(x2, 2.7557319e-6f, -1.9841270e-4f);
-    sn = fmaf(x2, sn, 8.3333333e-3f);
-    sn = fmaf(x2, sn, -1.6666667e-1f);
-    sn = fmaf(x2 * x, sn, x);
-    float cs = fmaf(x2, -2.7557319e-7f, 2.4801587e-5f);
-    cs = fmaf(x2, cs, -1.3888889e-3f);
-    cs = fmaf(x2, cs, 4.1666667e-2f);
-    cs = fmaf(x2, cs, -0.5f);
-    cs = fmaf(x2, cs, 1.0f);
-    const bool swap = ((o + 1) & 2) != 0;
-    const float a = swap ? cs : sn, b = swap ? sn : cs;
-    *s = (o & 4) ? -a : a;
-    *c = ((o + 2) & 4) ? -b : b;
+    *s = __builtin_amdgcn_sinf(u);
+    *c = __builtin_amdgcn_cosf(u);
 }
 
 // ---------------------------------------------------------------------------
@@ -355,8 +339,7 @@ __device__ __forceinline__ float normal_sample(const Stream& s, uint32_t sub) {
 }
 
 // Marsaglia-Tsang gamma(shape, scale). Each sampler block: a Box-Muller pair
-// (angle 2 pi u, exact octant reduction) and two acceptance uniforms (two
-// attempts). shape < 1 boosts Gamma(shape+1)
+// (angle 2 pi u, sincos_2pi) and two acceptance uniforms (two attempts). shape < 1 boosts Gamma(shape+1)
 // by U^(1/shape), U from the sub-stream's boost block.
 __device__ inline float gamma_sample(float shape, float scale, const Stream& s, uint32_t sub) {
     float boost = 1.0f;
