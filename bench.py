@@ -362,6 +362,16 @@ def main():
                               "instrumented kernel"},
     }
     dev.close()
+    # issue rates measured on this device (8 chains per lane, 8 waves per SIMD)
+    rates = {}
+    for key, which, scale in (("fma_f32_tflops", P._lib.MB_FMA_F32, 1e12),
+                              ("trans_f32_tops", P._lib.MB_TRANS_F32, 1e12),
+                              ("mad_u64_u32_tops", P._lib.MB_MAD_U64, 1e12)):
+        r = ctypes.c_double()
+        P._lib.check(lib.pp_microbench(0, which, ctypes.byref(r)), "pp_microbench")
+        rates[key] = r.value / scale
+    out["roofline"]["measured_issue_rates"] = rates
+    out["roofline"]["frac_of_measured_fma"] = achieved / rates["fma_f32_tflops"]
     out["surrogate"] = surrogate_leg(args) if args.surrogate else None
     if not args.no_cpu and args.workload == "cascade":
         out["cpu_baseline"] = cpu_baseline(

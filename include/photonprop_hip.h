@@ -221,6 +221,14 @@ int pp_device_memory(int device, int64_t* total_bytes, int64_t* free_bytes);
  * and compute-unit count of a device: the autotuner's cache key. */
 int pp_device_info(int device, char* arch, int arch_len, int32_t* n_cu);
 
+/* Vector-ALU issue rates of a device, for the roofline of the propagation
+ * kernel (an addition with no counterpart in the reference): 8 independent
+ * dependency chains per lane, 8 waves per SIMD, best of 3 timed launches.
+ * *rate: FP32 FLOP/s of v_fma_f32 (no packed FP32), or operations/s of the
+ * transcendental v_rcp_f32, or of v_mad_u64_u32 (the Philox product). */
+enum pp_microbench_kind { PP_MB_FMA_F32 = 0, PP_MB_TRANS_F32 = 1, PP_MB_MAD_U64 = 2 };
+int pp_microbench(int device, int which, double* rate);
+
 /* make_hit_buffers (src/photon_prop_cuda.jl:878-886): create a context on
  * `device` with a device hit buffer of `hit_capacity` records. The host
  * buffer stays with the caller (see pp_copy_hits). cfg may be NULL. */

@@ -50,6 +50,10 @@ SEARCH_AUTO = 0
 SEARCH_BRUTE = 1
 SEARCH_GRID = 2
 
+MB_FMA_F32 = 0
+MB_TRANS_F32 = 1
+MB_MAD_U64 = 2
+
 BATCH_AUTO = 0
 BATCH_PER_SOURCE = 1
 BATCH_PERSISTENT = 2
@@ -165,6 +169,7 @@ SIGNATURES = [
                                 c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_uint64,
                                 c_void_p, POINTER(c_int64)]),
     ("pp_device_info", c_int, [c_int, ctypes.c_char_p, c_int, POINTER(c_int32)]),
+    ("pp_microbench", c_int, [c_int, c_int, POINTER(ctypes.c_double)]),
     ("pp_get_config", c_int, [c_void_p, POINTER(pp_config)]),
     ("pp_set_config", c_int, [c_void_p, POINTER(pp_config)]),
     ("pp_mlp_create", c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int,

@@ -12,7 +12,7 @@ import subprocess
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 REPO = os.path.dirname(os.path.dirname(PKG_DIR))
-SOURCES = ["pp_lib.hip", "pp_mlp.hip", "pp_pmt.hip"]
+SOURCES = ["pp_lib.hip", "pp_mlp.hip", "pp_pmt.hip", "pp_bench.hip"]
 OUT = os.path.join(PKG_DIR, "libphotonprop_hip.so")
 
 HIPCC_FLAGS = [

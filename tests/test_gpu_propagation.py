@@ -419,3 +419,20 @@ def test_planar_targets_reject_grid_search_and_keep_batch_modes(P):
         assert g[1] == ref[1]
         for k in g[0]:
             assert g[0][k].tobytes() == ref[0][k].tobytes(), (mode, k)
+
+
+def test_microbench_issue_rates(P, gpu_lib):
+    """The roofline's measured issue rates: FP32 FMA near the non-packed
+    peak of MI355X (78.6 TFLOP/s) and never above it; the transcendental
+    and 64-bit integer products slower."""
+    import ctypes
+    r = {}
+    for k in (P._lib.MB_FMA_F32, P._lib.MB_TRANS_F32, P._lib.MB_MAD_U64):
+        v = ctypes.c_double()
+        assert gpu_lib.pp_microbench(0, k, ctypes.byref(v)) == 0
+        r[k] = v.value
+    print({k: f"{v / 1e12:.1f} T/s" for k, v in r.items()})
+    assert 30e12 < r[P._lib.MB_FMA_F32] <= 1.05 * 78.6e12
+    assert 0 < r[P._lib.MB_TRANS_F32] < r[P._lib.MB_FMA_F32] / 2
+    assert 0 < r[P._lib.MB_MAD_U64] < r[P._lib.MB_FMA_F32] / 2
+    assert gpu_lib.pp_microbench(0, 7, ctypes.byref(ctypes.c_double())) != 0
