@@ -85,6 +85,13 @@ int pp_mlp_set_params(pp_mlp* m, const float* params);
  * n x n_out); n <= max_batch. */
 int pp_mlp_forward(pp_mlp* m, const float* x, int n, float* y);
 
+/* Inference kernel choice (no counterpart in the reference: a tuning and
+ * test switch). fused = 1 (the default) runs every layer of a 64-row tile in
+ * one kernel with the activations kept in LDS, where the network has hidden
+ * width 256 and input and output widths <= 32; 0 runs one GEMM kernel per
+ * layer. Both give bitwise identical outputs. */
+int pp_mlp_set_fused(pp_mlp* m, int fused);
+
 /* Loss and gradient of one batch (host features n x n_in, labels n x
  * PP_SUR_LABEL_FLOATS): *loss and grads (pp_mlp_param_count floats, host)
  * of the mean loss. Does not change the parameters. */

@@ -142,6 +142,139 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference: the whole network for a tile of kFRows rows per
+// workgroup, the activations resident in LDS between layers (no HBM round
+// trip). The products are taken transposed, C^T = W . act^T: the weights are
+// MFMA operand A, held in registers and loaded straight from L2 in the packed
+// order of pack_fused_kernel (A row = output column n = lane%32 of the tile,
+// k = 16i + (lane/32)*8 + j; each wave load is 1 KB contiguous), the
+// activations operand B, read from LDS (B column = row m = lane%32). Wave w
+// owns output columns [64w, 64w+64) (two 32-column tiles), so every
+// activation fragment read from LDS feeds two MFMAs, and a lane's four
+// consecutive accumulator entries are four consecutive columns of one row:
+// the epilogue writes them with one 8-byte LDS store. Each output element is
+// the same 16-term MFMA dot products, K ascending with zero padding to a
+// multiple of 32, and the same bias/SiLU/bf16 epilogue as gemm_kernel.
+// Shapes: hidden width kFH, input and output widths <= 32, <= kFMaxLayers
+// weight matrices.
+// ---------------------------------------------------------------------------
+constexpr int kFH = 256;            // hidden width
+constexpr int kFRows = 64;          // rows per workgroup (two 32-row MFMA tiles)
+constexpr int kFThreads = 256;      // four waves
+constexpr int kFActLd = kFH + 8;    // LDS row stride of the activations (bf16)
+constexpr int kFFrag = kFH / 16;    // 16-deep K fragments of a hidden layer
+constexpr int kFMaxLayers = 8;
+static_assert(kFThreads == kFH, "one thread per bias entry of a layer");
+
+struct FusedArgs {
+    const bf16* X;   // rows x n_in (bf16)
+    int rows, n_in, n_out, n_mat;
+    const bf16* W[kFMaxLayers];    // layer l packed by pack_fused_kernel
+    const float* b[kFMaxLayers];
+    float* Y;        // rows x n_out (fp32)
+};
+
+enum FusedLayer { kFInput = 0, kFHidden = 1, kFOutput = 2 };
+
+// One layer for wave w: weights into registers, barrier (the layer's input
+// activations in `in` complete), products, epilogue into `out` (bf16 SiLU
+// activations, row stride kFActLd) or, for the output layer, fp32 outputs
+// into `out` viewed as float[kFRows][32]. Every wave calls it (the barrier);
+// in the output layer (N <= 32) only wave 0 computes.
+template <int MODE>
+__device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, const bf16* in, bf16* out,
+                                            const float* bias_l, int lane, int w) {
+    constexpr int NT = MODE == kFOutput ? 1 : 2;          // 32-column tiles per wave
+    constexpr int NF = MODE == kFInput ? 2 : kFFrag;      // K fragments
+    const bool active = MODE != kFOutput || w == 0;
+    const int kl = (lane >> 5) * 8;
+    bf16x8 wa[NT][NF];
+    if (active) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x8* src = reinterpret_cast<const bf16x8*>(f.W[l]) +
+                                (size_t)(2 * w + nt) * NF * 64 + lane;
+#pragma unroll
+            for (int i = 0; i < NF; ++i) wa[nt][i] = src[64 * i];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    f32x16 acc[NT][2];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt][0] = acc[nt][1] = f32x16{};
+    const bf16* b_lo = in + (lane & 31) * kFActLd + kl;
+    const bf16* b_hi = b_lo + 32 * kFActLd;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(b_lo + 16 * i);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(b_hi + 16 * i);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            acc[nt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b0, acc[nt][0], 0, 0, 0);
+            acc[nt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b1, acc[nt][1], 0, 0, 0);
+        }
+    }
+    // acc[nt][mt][4g + j]: column 64w + 32nt + 8g + (lane/32)*4 + j, row 32mt + lane%32
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int n0 = 64 * w + 32 * nt + 8 * g + (lane >> 5) * 4;
+            const float4 b4 = *reinterpret_cast<const float4*>(bias_l + n0);
+            const float bias[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int m = 32 * mt + (lane & 31);
+                float z[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) z[j] = acc[nt][mt][4 * g + j] + bias[j];
+                if (MODE == kFOutput) {
+                    *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + m * 32 + n0) =
+                        make_float4(z[0], z[1], z[2], z[3]);
+                } else {
+                    bf16 h[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) h[j] = (bf16)(z[j] * sigmoid(z[j]));
+                    *reinterpret_cast<uint2*>(out + m * kFActLd + n0) =
+                        *reinterpret_cast<const uint2*>(h);
+                }
+            }
+        }
+    }
+}
+
+// two workgroups per CU: LDS 75.8 KB each, <= 256 VGPRs per lane
+__global__ void __launch_bounds__(kFThreads, 2) mlp_fused_kernel(FusedArgs f) {
+    __shared__ __attribute__((aligned(16))) bf16 act[2][kFRows * kFActLd];
+    __shared__ __attribute__((aligned(16))) float bias[kFMaxLayers][kFH];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int row0 = blockIdx.x * kFRows;
+    // every layer's biases, zero beyond the layer's width
+    for (int l = 0; l < f.n_mat; ++l)
+        bias[l][t] = t < (l + 1 == f.n_mat ? f.n_out : kFH) ? f.b[l][t] : 0.0f;
+    // input tile, zero beyond n_in (up to 32 columns) and beyond the last row
+    for (int i = t; i < kFRows * 32; i += kFThreads) {
+        const int r = i >> 5, k = i & 31;
+        const int gr = row0 + r;
+        act[0][r * kFActLd + k] = (gr < f.rows && k < f.n_in) ? f.X[(size_t)gr * f.n_in + k]
+                                                              : (bf16)0.0f;
+    }
+    fused_layer<kFInput>(f, 0, act[0], act[1], bias[0], lane, w);
+    int cur = 1;
+    for (int l = 1; l + 1 < f.n_mat; ++l, cur ^= 1)
+        fused_layer<kFHidden>(f, l, act[cur], act[cur ^ 1], bias[l], lane, w);
+    fused_layer<kFOutput>(f, f.n_mat - 1, act[cur], act[cur ^ 1], bias[f.n_mat - 1], lane, w);
+    __syncthreads();
+    const float* zs = reinterpret_cast<const float*>(act[cur ^ 1]);
+    const int valid_rows = f.rows - row0 < kFRows ? f.rows - row0 : kFRows;
+    for (int i = t; i < valid_rows * f.n_out; i += kFThreads) {
+        const int r = i / f.n_out, c = i - r * f.n_out;
+        f.Y[(size_t)row0 * f.n_out + i] = zs[r * 32 + c];
+    }
+}
+
 // Column sums of X (rows x cols, bf16) in row slices: blockIdx.y sums rows
 // [y*row_chunk, (y+1)*row_chunk) into part[y*cols + c] (bias gradients,
 // finished by reduce_partials_kernel in a fixed order).
@@ -307,6 +440,22 @@ __global__ void bump_kernel(uint32_t* step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
+// Weights of one layer in the order the fused inference kernel loads them:
+// for 32-column tile T and 16-deep K fragment i, the 64 lanes' 8 bf16 each,
+// contiguous: chunk (T*nf + i)*64 + lane holds W[32T + lane%32][16i +
+// (lane/32)*8 + j], j = 0..7, zero beyond K and N.
+__global__ void pack_fused_kernel(const bf16* W, int K, int N, int nf, int64_t chunks,
+                                  bf16* out) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= chunks) return;
+    const int lane = (int)(c & 63);
+    const int64_t ti = c >> 6;
+    const int i = (int)(ti % nf), T = (int)(ti / nf);
+    const int n = 32 * T + (lane & 31), k0 = 16 * i + (lane >> 5) * 8;
+    for (int j = 0; j < 8; ++j)
+        out[c * 8 + j] = (n < N && k0 + j < K) ? W[(size_t)n * K + k0 + j] : (bf16)0.0f;
+}
+
 __global__ void to_bf16_kernel(const float* x, bf16* y, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = (bf16)x[i];
@@ -439,6 +588,12 @@ struct pp_mlp {
     // parameters: internal layout with every block at a 64-element boundary
     std::vector<int64_t> w_off, b_off;
     int64_t n_int = 0, n_params = 0;
+    bool fused = true;  // inference through mlp_fused_kernel where the shape allows
+    // the fused kernel's copy of the bf16 weights (pack_fused_kernel order),
+    // refreshed by the first inference after the weights change
+    bf16* wpk = nullptr;
+    std::vector<int64_t> pk_off;
+    bool pk_ok = false;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16* pb = nullptr;
     // activations (max_batch rows): H[l] bf16 with dims[l] columns (H[0] = the
@@ -483,9 +638,50 @@ int gemm(pp_mlp* M, const GemmArgs& a) {
     return PP_OK;
 }
 
+// The fused inference kernel takes hidden layers of width kFH and input and
+// output widths of at most 32.
+bool fused_ok(const pp_mlp* M) {
+    if (M->n_mat > kFMaxLayers || M->dims.front() > 32 || M->dims.back() > 32) return false;
+    for (size_t i = 1; i + 1 < M->dims.size(); ++i)
+        if (M->dims[i] != kFH) return false;
+    return true;
+}
+
+int forward_fused(pp_mlp* M, int rows) {
+    if (!M->pk_ok) {
+        for (int l = 0; l < M->n_mat; ++l) {
+            const int K = M->dims[l], N = M->dims[l + 1];
+            const int nf = (K + 31) / 32 * 2;
+            const int64_t chunks = (M->pk_off[l + 1] - M->pk_off[l]) / 8;
+            hipLaunchKernelGGL(pack_fused_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0,
+                               M->stream, M->pb + M->w_off[l], K, N, nf, chunks,
+                               M->wpk + M->pk_off[l]);
+            PP_HIP(hipGetLastError());
+        }
+        M->pk_ok = true;
+    }
+    FusedArgs f{};
+    f.X = M->H[0];
+    f.rows = rows;
+    f.n_in = M->dims.front();
+    f.n_out = M->dims.back();
+    f.n_mat = M->n_mat;
+    for (int l = 0; l < M->n_mat; ++l) {
+        f.W[l] = M->wpk + M->pk_off[l];
+        f.b[l] = M->p + M->b_off[l];
+    }
+    f.Y = M->Y;
+    hipLaunchKernelGGL(mlp_fused_kernel, dim3(blocks_for(rows, kFRows)), dim3(kFThreads), 0,
+                       M->stream, f);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
+}
+
 // forward of `rows` rows whose bf16 input is in H[0]; keep_z stores the fp32
-// pre-activations that the backward pass needs
+// pre-activations that the backward pass needs (and uses the layer-by-layer
+// kernels); inference runs the fused kernel where the shape allows
 int forward(pp_mlp* M, int rows, bool keep_z) {
+    if (!keep_z && M->fused && M->wpk) return forward_fused(M, rows);
     for (int l = 0; l < M->n_mat; ++l) {
         GemmArgs a{};
         a.A = M->H[l];
@@ -597,6 +793,7 @@ int backward(pp_mlp* M, int rows) {
 }
 
 int adam(pp_mlp* M, const pp_adam& o) {
+    M->pk_ok = false;
     AdamArgs a{M->p, M->g, M->m, M->v, M->pb, M->n_int, o, M->step};
     hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(M->n_int, 256)), dim3(256), 0, M->stream, a);
     PP_HIP(hipGetLastError());
@@ -630,6 +827,7 @@ void unpack_params(const pp_mlp* M, const float* in, std::vector<float>& out) {
 }
 
 int upload_params(pp_mlp* M, const std::vector<float>& internal) {
+    M->pk_ok = false;
     PP_HIP(hipMemcpyAsync(M->p, internal.data(), M->n_int * 4, hipMemcpyHostToDevice, M->stream));
     PP_HIP(hipMemsetAsync(M->m, 0, M->n_int * 4, M->stream));
     PP_HIP(hipMemsetAsync(M->v, 0, M->n_int * 4, M->stream));
@@ -731,6 +929,13 @@ int pp_mlp_create(pp_mlp** out, int device, int n_in, int n_hidden, int n_layers
         ok = alloc((void**)&M->H[l], rows * M->dims[l] * 2) == hipSuccess;
         if (ok && l > 0) ok = alloc((void**)&M->Z[l], rows * M->dims[l] * 4) == hipSuccess;
     }
+    // packed weights of the fused inference kernel: 32-column tiles x 16-deep
+    // K fragments (K padded to a multiple of 32) x 512 bf16
+    M->pk_off.assign(1, 0);
+    for (int l = 0; l < M->n_mat; ++l)
+        M->pk_off.push_back(M->pk_off.back() + (int64_t)((M->dims[l + 1] + 31) / 32) *
+                                                   ((M->dims[l] + 31) / 32 * 2) * 512);
+    if (ok && fused_ok(M)) ok = alloc((void**)&M->wpk, M->pk_off.back() * 2) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
         return bail(fail(PP_ERR_NOMEM, "pp_mlp_create: device allocation failed"));
@@ -774,7 +979,7 @@ int pp_mlp_destroy(pp_mlp* M) {
     (void)hipSetDevice(M->device);
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     void* ptrs[] = {M->p, M->g, M->m, M->v, M->pb, M->Y, M->dY, M->dYb, M->dA, M->dB, M->Lb, M->xs,
-                    M->part,
+                    M->part, M->wpk,
                     M->loss, M->step, M->dsX, M->dsL, M->logy, M->hits, M->mix, M->fb,
                     M->d_ev, M->d_mod};
     for (void* p : ptrs)
@@ -791,6 +996,12 @@ int pp_mlp_destroy(pp_mlp* M) {
 }
 
 int64_t pp_mlp_param_count(pp_mlp* M) { return M ? M->n_params : -1; }
+
+int pp_mlp_set_fused(pp_mlp* M, int fused) {
+    if (!M) return fail(PP_ERR_INVALID, "pp_mlp_set_fused: NULL model");
+    M->fused = fused != 0;
+    return PP_OK;
+}
 
 int pp_mlp_get_params(pp_mlp* M, float* params) {
     if (!M || !params) return fail(PP_ERR_INVALID, "pp_mlp_get_params: NULL argument");
@@ -862,6 +1073,7 @@ int pp_mlp_train(pp_mlp* M, const float* x, const float* labels, int64_t n, int 
     if (n < 1 || batch < 1 || batch > M->max_batch || n_steps < 0)
         return fail(PP_ERR_INVALID, "pp_mlp_train: bad sizes");
     if (int rc = set_dev(M)) return rc;
+    M->pk_ok = false;  // the replayed steps update the weights
     const int n_in = M->dims[0];
     if (n > M->ds_cap) {
         if (M->dsX) PP_HIP(hipFree(M->dsX));

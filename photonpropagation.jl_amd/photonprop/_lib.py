@@ -179,6 +179,7 @@ SIGNATURES = [
     ("pp_mlp_get_params", c_int, [c_void_p, c_void_p]),
     ("pp_mlp_set_params", c_int, [c_void_p, c_void_p]),
     ("pp_mlp_forward", c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    ("pp_mlp_set_fused", c_int, [c_void_p, c_int]),
     ("pp_mlp_loss_grad", c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(c_float),
                                  c_void_p]),
     ("pp_mlp_adam_step", c_int, [c_void_p, POINTER(pp_adam)]),

@@ -96,6 +96,11 @@ class MLP:
                    "pp_mlp_forward")
         return y
 
+    def set_fused(self, fused=True):
+        """Inference through the fused all-layer kernel (default) or one GEMM
+        kernel per layer; the outputs are bitwise identical."""
+        _lib.check(self.lib.pp_mlp_set_fused(self.handle, int(bool(fused))), "pp_mlp_set_fused")
+
     def loss_grad(self, x, labels):
         x = np.ascontiguousarray(x, np.float32)
         labels = np.ascontiguousarray(labels, np.float32)

@@ -56,6 +56,48 @@ def test_mfma_forward_matches_oracle(S, O):
             assert np.all(err <= 2e-3 + 2e-3 * np.abs(yo))
 
 
+@pytest.mark.parametrize("n_in,n_out", [(8, 10), (24, 32), (1, 10)])
+def test_fused_inference_equals_layered(S, O, n_in, n_out):
+    """The all-layer kernel and the per-layer GEMM kernels give the same bits
+    (same MFMA order, zero padding and epilogue), on partial and full 64-row
+    tiles, and both agree with the oracle."""
+    rng = np.random.default_rng(11 + n_in)
+    with S.MLP(n_in=n_in, n_out=n_out, max_batch=4096, seed=5) as m:
+        p = m.get_params()
+        p[:] = rng.normal(0, 0.08, p.size)  # bias terms nonzero too
+        m.set_params(p)
+        for n in (1, 63, 65, 300, 4096):
+            x = np.clip(rng.normal(0, 1, (n, n_in)), -3, 3).astype(np.float32)
+            m.set_fused(True)
+            yf = m.forward(x)
+            m.set_fused(False)
+            yl = m.forward(x)
+            assert np.array_equal(yf.view(np.uint32), yl.view(np.uint32)), n
+            yo = O.mlp_forward(m.dims, p, x, quant=True)
+            assert np.all(np.abs(yf - yo) <= 2e-3 + 2e-3 * np.abs(yo))
+
+
+def test_fused_pair_evaluation_equals_layered(S):
+    rng = np.random.default_rng(13)
+    ne, nm = 300, 500
+    ev = np.zeros((ne, 8), np.float32)
+    ev[:, :3] = rng.uniform(-300, 300, (ne, 3))
+    d = rng.normal(size=(ne, 3))
+    ev[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    ev[:, 6] = np.exp(rng.uniform(np.log(200), np.log(5e5), ne))
+    ev[:, 7] = rng.uniform(15, 25, ne)
+    mods = rng.uniform(-300, 300, (nm, 3)).astype(np.float32)
+    dom = S.default_setup().domain()
+    with S.MLP(max_batch=1 << 16, seed=10) as m:
+        out = {}
+        for fused in (False, True, False, True):  # second pass of each is the timed one
+            m.set_fused(fused)
+            out[fused] = m.eval_pairs(ev, mods, dom, mixture=True)
+        (hl, ml, fl, tl), (hf, mf, ff, tf) = out[False], out[True]
+        print(f"{ne * nm} pairs: layered {tl:.3f} ms, fused {tf:.3f} ms")
+        assert np.array_equal(hf, hl) and np.array_equal(mf, ml) and np.array_equal(ff, fl)
+
+
 def test_gradients_match_oracle(S, O):
     rng = np.random.default_rng(2)
     with S.MLP(max_batch=1024, seed=4) as m:
