@@ -56,7 +56,11 @@ struct GemmArgs {
     int64_t c_stride;
 };
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function from the hardware exp2 and reciprocal (each about 1 ulp);
+// the activations it feeds are rounded to bf16 (8-bit significand)
+__device__ __forceinline__ float sigmoid(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.44269504f));
+}
 
 // 8 consecutive bf16 at p[0..7] (zero where i >= valid) into dst[0..7]
 __device__ __forceinline__ void load8(const bf16* p, int valid, bool vec, bf16* dst) {
@@ -150,17 +154,19 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
 // order of pack_fused_kernel (A row = output column n = lane%32 of the tile,
 // k = 16i + (lane/32)*8 + j; each wave load is 1 KB contiguous), the
 // activations operand B, read from LDS (B column = row m = lane%32). Wave w
-// owns output columns [64w, 64w+64) (two 32-column tiles), so every
-// activation fragment read from LDS feeds two MFMAs, and a lane's four
-// consecutive accumulator entries are four consecutive columns of one row:
-// the epilogue writes them with one 8-byte LDS store. Each output element is
-// the same 16-term MFMA dot products, K ascending with zero padding to a
+// owns output columns [64w, 64w+64) (two 32-column tiles) of all kFRows rows
+// (four 32-row tiles): each weight fragment feeds four MFMAs and each
+// activation fragment two, and a lane's four consecutive accumulator entries
+// are four consecutive columns of one row, written with one 8-byte LDS store.
+// One activation buffer serves as a layer's input and output (a barrier
+// between the last read and the first write). Each output element is the
+// same 16-term MFMA dot products, K ascending with zero padding to a
 // multiple of 32, and the same bias/SiLU/bf16 epilogue as gemm_kernel.
 // Shapes: hidden width kFH, input and output widths <= 32, <= kFMaxLayers
 // weight matrices.
 // ---------------------------------------------------------------------------
 constexpr int kFH = 256;            // hidden width
-constexpr int kFRows = 64;          // rows per workgroup (two 32-row MFMA tiles)
+constexpr int kFRows = 128;         // rows per workgroup (four 32-row MFMA tiles)
 constexpr int kFThreads = 256;      // four waves
 constexpr int kFActLd = kFH + 8;    // LDS row stride of the activations (bf16)
 constexpr int kFFrag = kFH / 16;    // 16-deep K fragments of a hidden layer
@@ -177,77 +183,85 @@ struct FusedArgs {
 
 enum FusedLayer { kFInput = 0, kFHidden = 1, kFOutput = 2 };
 
-// One layer for wave w: weights into registers, barrier (the layer's input
-// activations in `in` complete), products, epilogue into `out` (bf16 SiLU
-// activations, row stride kFActLd) or, for the output layer, fp32 outputs
-// into `out` viewed as float[kFRows][32]. Every wave calls it (the barrier);
-// in the output layer (N <= 32) only wave 0 computes.
+// One layer for wave w: products over K in batches of eight weight
+// fragments, barrier (every wave's reads of the layer input done), epilogue
+// into `act` (bf16 SiLU activations, row stride kFActLd) or, for the output
+// layer, fp32 outputs into `act` viewed as float[kFRows][32], barrier. In the
+// output layer (N <= 32, one column tile) wave w takes row tile w.
 template <int MODE>
-__device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, const bf16* in, bf16* out,
+__device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act,
                                             const float* bias_l, int lane, int w) {
     constexpr int NT = MODE == kFOutput ? 1 : 2;          // 32-column tiles per wave
+    constexpr int MT = MODE == kFOutput ? 1 : 4;          // 32-row tiles per wave
     constexpr int NF = MODE == kFInput ? 2 : kFFrag;      // K fragments
-    const bool active = MODE != kFOutput || w == 0;
+    constexpr int NB = NF < 8 ? NF : 8;                   // fragments per load batch
     const int kl = (lane >> 5) * 8;
-    bf16x8 wa[NT][NF];
-    if (active) {
+    const int t0 = MODE == kFOutput ? 0 : 2 * w;          // first column tile
+    const int m0 = MODE == kFOutput ? 32 * w : 0;         // first row
+    const bf16x8* wsrc = reinterpret_cast<const bf16x8*>(f.W[l]) + (size_t)t0 * NF * 64 + lane;
+    const bf16* bsrc = act + (m0 + (lane & 31)) * kFActLd + kl;
+    f32x16 acc[NT][MT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const bf16x8* src = reinterpret_cast<const bf16x8*>(f.W[l]) +
-                                (size_t)(2 * w + nt) * NF * 64 + lane;
+    for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-            for (int i = 0; i < NF; ++i) wa[nt][i] = src[64 * i];
+        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = f32x16{};
+#pragma unroll 1
+    for (int i0 = 0; i0 < NF; i0 += NB) {
+        bf16x8 wa[NT][NB];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) wa[nt][i] = wsrc[(nt * NF + i0 + i) * 64];
+        // the batch's loads all in flight before the first product waits on one
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            bf16x8 b[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                b[mt] = *reinterpret_cast<const bf16x8*>(bsrc + 32 * mt * kFActLd + 16 * (i0 + i));
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[nt][mt] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b[mt], acc[nt][mt], 0, 0, 0);
         }
     }
     __syncthreads();
-    if (!active) return;
-    f32x16 acc[NT][2];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[nt][0] = acc[nt][1] = f32x16{};
-    const bf16* b_lo = in + (lane & 31) * kFActLd + kl;
-    const bf16* b_hi = b_lo + 32 * kFActLd;
-#pragma unroll
-    for (int i = 0; i < NF; ++i) {
-        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(b_lo + 16 * i);
-        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(b_hi + 16 * i);
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            acc[nt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b0, acc[nt][0], 0, 0, 0);
-            acc[nt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b1, acc[nt][1], 0, 0, 0);
-        }
-    }
-    // acc[nt][mt][4g + j]: column 64w + 32nt + 8g + (lane/32)*4 + j, row 32mt + lane%32
+    // acc[nt][mt][4g + j]: column 32(t0 + nt) + 8g + (lane/32)*4 + j, row m0 + 32mt + lane%32
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const int n0 = 64 * w + 32 * nt + 8 * g + (lane >> 5) * 4;
+            const int n0 = 32 * (t0 + nt) + 8 * g + (lane >> 5) * 4;
             const float4 b4 = *reinterpret_cast<const float4*>(bias_l + n0);
             const float bias[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const int m = 32 * mt + (lane & 31);
+            for (int mt = 0; mt < MT; ++mt) {
+                const int m = m0 + 32 * mt + (lane & 31);
                 float z[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) z[j] = acc[nt][mt][4 * g + j] + bias[j];
                 if (MODE == kFOutput) {
-                    *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + m * 32 + n0) =
+                    *reinterpret_cast<float4*>(reinterpret_cast<float*>(act) + m * 32 + n0) =
                         make_float4(z[0], z[1], z[2], z[3]);
                 } else {
                     bf16 h[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) h[j] = (bf16)(z[j] * sigmoid(z[j]));
-                    *reinterpret_cast<uint2*>(out + m * kFActLd + n0) =
+                    *reinterpret_cast<uint2*>(act + m * kFActLd + n0) =
                         *reinterpret_cast<const uint2*>(h);
                 }
             }
         }
     }
+    __syncthreads();
 }
 
 // two workgroups per CU: LDS 75.8 KB each, <= 256 VGPRs per lane
 __global__ void __launch_bounds__(kFThreads, 2) mlp_fused_kernel(FusedArgs f) {
-    __shared__ __attribute__((aligned(16))) bf16 act[2][kFRows * kFActLd];
+    __shared__ __attribute__((aligned(16))) bf16 act[kFRows * kFActLd];
     __shared__ __attribute__((aligned(16))) float bias[kFMaxLayers][kFH];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int row0 = blockIdx.x * kFRows;
@@ -258,16 +272,14 @@ __global__ void __launch_bounds__(kFThreads, 2) mlp_fused_kernel(FusedArgs f) {
     for (int i = t; i < kFRows * 32; i += kFThreads) {
         const int r = i >> 5, k = i & 31;
         const int gr = row0 + r;
-        act[0][r * kFActLd + k] = (gr < f.rows && k < f.n_in) ? f.X[(size_t)gr * f.n_in + k]
-                                                              : (bf16)0.0f;
+        act[r * kFActLd + k] = (gr < f.rows && k < f.n_in) ? f.X[(size_t)gr * f.n_in + k]
+                                                           : (bf16)0.0f;
     }
-    fused_layer<kFInput>(f, 0, act[0], act[1], bias[0], lane, w);
-    int cur = 1;
-    for (int l = 1; l + 1 < f.n_mat; ++l, cur ^= 1)
-        fused_layer<kFHidden>(f, l, act[cur], act[cur ^ 1], bias[l], lane, w);
-    fused_layer<kFOutput>(f, f.n_mat - 1, act[cur], act[cur ^ 1], bias[f.n_mat - 1], lane, w);
     __syncthreads();
-    const float* zs = reinterpret_cast<const float*>(act[cur ^ 1]);
+    fused_layer<kFInput>(f, 0, act, bias[0], lane, w);
+    for (int l = 1; l + 1 < f.n_mat; ++l) fused_layer<kFHidden>(f, l, act, bias[l], lane, w);
+    fused_layer<kFOutput>(f, f.n_mat - 1, act, bias[f.n_mat - 1], lane, w);
+    const float* zs = reinterpret_cast<const float*>(act);
     const int valid_rows = f.rows - row0 < kFRows ? f.rows - row0 : kFRows;
     for (int i = t; i < valid_rows * f.n_out; i += kFThreads) {
         const int r = i / f.n_out, c = i - r * f.n_out;
