@@ -189,10 +189,24 @@ __device__ __forceinline__ float sl_cos(const Phase& p, float u) {
     return clamp1(2.0f * exp2f(p.sl_exp * log2f(u)) - 1.0f);
 }
 
+// The HG/ES mixture without a branch: nearly every wave holds photons of both
+// kinds, so both samplers run anyway; they share the one Newton reciprocal,
+// of HG's denominator or of ES's root w (the same values as hg_cos and es_cos).
+__device__ __forceinline__ float hg_es_cos(const Phase& p, float u_choice, float u) {
+    const bool hg = u_choice < p.hg_fraction;
+    const float q = p.es_Q * (1.0f - 2.0f * u);
+    const float hq = 0.5f * fabsf(q);
+    const float w = cbrt_pos(hq + sqrt_hw(hq * hq + p.es_P3cube));
+    const float r = rcp_nr(hg ? p.hg_D0 + p.hg_D1 * u : w);
+    const float t = p.hg_B * r;
+    const float x = w - p.es_P3 * r;
+    return clamp1(hg ? p.hg_A - p.hg_C * (t * t) : (q > 0.0f ? -x : x));
+}
+
 __device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float u) {
     if (p.kind == PP_PHASE_HG) return hg_cos(p, u);
     if (p.kind == PP_PHASE_MIXED_HG_SL) return u_choice < p.hg_fraction ? hg_cos(p, u) : sl_cos(p, u);
-    return u_choice < p.hg_fraction ? hg_cos(p, u) : es_cos(p, u);
+    return hg_es_cos(p, u_choice, u);
 }
 
 // update_direction   src/photon_prop_cuda.jl:280-298
