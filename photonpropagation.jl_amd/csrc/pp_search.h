@@ -174,13 +174,16 @@ struct GridSearch {
         // first, the next index (wrapped into another row, or clamped into the
         // table) names a cell off the segment, whose targets are tested
         // exactly and cannot change the smallest hit index.
-        const int dcx = sx, dcy = sy * g.nx, dcz = sz * g.nx * g.ny;
-        const int c_max = g.n_cells - 1;
-        int c = (cz * g.ny + cy) * g.nx + cx;
+        // the cell as the byte offset of its record pair (32 bytes a cell)
+        const int dcx = 32 * sx, dcy = 32 * sy * g.nx, dcz = 32 * sz * g.nx * g.ny;
+        const int c_max = 32 * (g.n_cells - 1);
+        const char* const rec = reinterpret_cast<const char*>(cells);
+        int c = 32 * ((cz * g.ny + cy) * g.nx + cx);
         const int max_visits = g.nx + g.ny + g.nz;
         for (int v = 0; v < max_visits; ++v) {
             // this cell's record; the walk's next step is computed while it loads
-            const float4 lo = cells[2 * c], hi = cells[2 * c + 1];
+            const float4 lo = *reinterpret_cast<const float4*>(rec + c);
+            const float4 hi = *reinterpret_cast<const float4*>(rec + c + 16);
             // axis of the nearest boundary (ties: x, then y), by compares and
             // selects rather than fminf, whose NaN rules cost a canonicalising
             // instruction per operand here
@@ -193,15 +196,29 @@ struct GridSearch {
             if (COUNT) w.cells += 1;
             // slab test of the segment [0, step] against the box of this
             // cell's targets (an empty cell holds a box at +inf, which fails
-            // it), then the targets themselves
+            // it), then the targets themselves. In one layer of columns the
+            // boxes' z slab is left out: the clip already holds the segment
+            // inside the grid's z range, a column's box spans its string,
+            // and a wider [b_in, b_out] only admits candidates that the z
+            // filter and the exact sphere test below then settle.
             const float ax0 = fmaf(lo.x, ix_, -px_), ax1 = fmaf(hi.x, ix_, -px_);
             const float ay0 = fmaf(lo.y, iy_, -py_), ay1 = fmaf(hi.y, iy_, -py_);
-            const float az0 = fmaf(lo.z, iz_, -pz_), az1 = fmaf(hi.z, iz_, -pz_);
-            const float b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)),
-                                     fmaxf(fminf(az0, az1), 0.0f));
-            const float b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)),
-                                      fminf(fmaxf(az0, az1), step));
-            if (b_in <= b_out) {
+            float b_in, b_out;
+            bool pass;
+            if (COLS) {
+                // the end of the segment as a compare of its own (a minimum
+                // with step would canonicalise step in every iteration)
+                b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), 0.0f);
+                b_out = fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1));
+                pass = b_in <= b_out && b_in <= step;
+            } else {
+                const float az0 = fmaf(lo.z, iz_, -pz_), az1 = fmaf(hi.z, iz_, -pz_);
+                b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), 0.0f));
+                b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), step));
+                pass = b_in <= b_out;
+            }
+            if (pass) {
+                if (COLS) b_out = fminf(b_out, step);
                 const uint32_t b = float_bits(lo.w), cnt = float_bits(hi.w);
                 const float exc = dir_excess(dir);  // rarely needed: not kept across the walk
                 // a target is hit inside the box, so within its radius of the
