@@ -146,147 +146,6 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Fused inference: the whole network for a tile of kFRows rows per
-// workgroup, the activations resident in LDS between layers (no HBM round
-// trip). The products are taken transposed, C^T = W . act^T: the weights are
-// MFMA operand A, held in registers and loaded straight from L2 in the packed
-// order of pack_fused_kernel (A row = output column n = lane%32 of the tile,
-// k = 16i + (lane/32)*8 + j; each wave load is 1 KB contiguous), the
-// activations operand B, read from LDS (B column = row m = lane%32). Wave w
-// owns output columns [64w, 64w+64) (two 32-column tiles) of all kFRows rows
-// (four 32-row tiles): each weight fragment feeds four MFMAs and each
-// activation fragment two, and a lane's four consecutive accumulator entries
-// are four consecutive columns of one row, written with one 8-byte LDS store.
-// One activation buffer serves as a layer's input and output (a barrier
-// between the last read and the first write). Each output element is the
-// same 16-term MFMA dot products, K ascending with zero padding to a
-// multiple of 32, and the same bias/SiLU/bf16 epilogue as gemm_kernel.
-// Shapes: hidden width kFH, input and output widths <= 32, <= kFMaxLayers
-// weight matrices.
-// ---------------------------------------------------------------------------
-constexpr int kFH = 256;            // hidden width
-constexpr int kFRows = 128;         // rows per workgroup (four 32-row MFMA tiles)
-constexpr int kFThreads = 256;      // four waves
-constexpr int kFActLd = kFH + 8;    // LDS row stride of the activations (bf16)
-constexpr int kFFrag = kFH / 16;    // 16-deep K fragments of a hidden layer
-constexpr int kFMaxLayers = 8;
-static_assert(kFThreads == kFH, "one thread per bias entry of a layer");
-
-struct FusedArgs {
-    const bf16* X;   // rows x n_in (bf16)
-    int rows, n_in, n_out, n_mat;
-    const bf16* W[kFMaxLayers];    // layer l packed by pack_fused_kernel
-    const float* b[kFMaxLayers];
-    float* Y;        // rows x n_out (fp32)
-};
-
-enum FusedLayer { kFInput = 0, kFHidden = 1, kFOutput = 2 };
-
-// One layer for wave w: products over K in batches of eight weight
-// fragments, barrier (every wave's reads of the layer input done), epilogue
-// into `act` (bf16 SiLU activations, row stride kFActLd) or, for the output
-// layer, fp32 outputs into `act` viewed as float[kFRows][32], barrier. In the
-// output layer (N <= 32, one column tile) wave w takes row tile w.
-template <int MODE>
-__device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act,
-                                            const float* bias_l, int lane, int w) {
-    constexpr int NT = MODE == kFOutput ? 1 : 2;          // 32-column tiles per wave
-    constexpr int MT = MODE == kFOutput ? 1 : 4;          // 32-row tiles per wave
-    constexpr int NF = MODE == kFInput ? 2 : kFFrag;      // K fragments
-    constexpr int NB = NF < 8 ? NF : 8;                   // fragments per load batch
-    const int kl = (lane >> 5) * 8;
-    const int t0 = MODE == kFOutput ? 0 : 2 * w;          // first column tile
-    const int m0 = MODE == kFOutput ? 32 * w : 0;         // first row
-    const bf16x8* wsrc = reinterpret_cast<const bf16x8*>(f.W[l]) + (size_t)t0 * NF * 64 + lane;
-    const bf16* bsrc = act + (m0 + (lane & 31)) * kFActLd + kl;
-    f32x16 acc[NT][MT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = f32x16{};
-#pragma unroll 1
-    for (int i0 = 0; i0 < NF; i0 += NB) {
-        bf16x8 wa[NT][NB];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-            for (int i = 0; i < NB; ++i) wa[nt][i] = wsrc[(nt * NF + i0 + i) * 64];
-        // the batch's loads all in flight before the first product waits on one
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            bf16x8 b[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-                b[mt] = *reinterpret_cast<const bf16x8*>(bsrc + 32 * mt * kFActLd + 16 * (i0 + i));
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                    acc[nt][mt] =
-                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b[mt], acc[nt][mt], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-    // acc[nt][mt][4g + j]: column 32(t0 + nt) + 8g + (lane/32)*4 + j, row m0 + 32mt + lane%32
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int n0 = 32 * (t0 + nt) + 8 * g + (lane >> 5) * 4;
-            const float4 b4 = *reinterpret_cast<const float4*>(bias_l + n0);
-            const float bias[4] = {b4.x, b4.y, b4.z, b4.w};
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int m = m0 + 32 * mt + (lane & 31);
-                float z[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) z[j] = acc[nt][mt][4 * g + j] + bias[j];
-                if (MODE == kFOutput) {
-                    *reinterpret_cast<float4*>(reinterpret_cast<float*>(act) + m * 32 + n0) =
-                        make_float4(z[0], z[1], z[2], z[3]);
-                } else {
-                    bf16 h[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) h[j] = (bf16)(z[j] * sigmoid(z[j]));
-                    *reinterpret_cast<uint2*>(act + m * kFActLd + n0) =
-                        *reinterpret_cast<const uint2*>(h);
-                }
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// two workgroups per CU: LDS 75.8 KB each, <= 256 VGPRs per lane
-__global__ void __launch_bounds__(kFThreads, 2) mlp_fused_kernel(FusedArgs f) {
-    __shared__ __attribute__((aligned(16))) bf16 act[kFRows * kFActLd];
-    __shared__ __attribute__((aligned(16))) float bias[kFMaxLayers][kFH];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int row0 = blockIdx.x * kFRows;
-    // every layer's biases, zero beyond the layer's width
-    for (int l = 0; l < f.n_mat; ++l)
-        bias[l][t] = t < (l + 1 == f.n_mat ? f.n_out : kFH) ? f.b[l][t] : 0.0f;
-    // input tile, zero beyond n_in (up to 32 columns) and beyond the last row
-    for (int i = t; i < kFRows * 32; i += kFThreads) {
-        const int r = i >> 5, k = i & 31;
-        const int gr = row0 + r;
-        act[r * kFActLd + k] = (gr < f.rows && k < f.n_in) ? f.X[(size_t)gr * f.n_in + k]
-                                                           : (bf16)0.0f;
-    }
-    __syncthreads();
-    fused_layer<kFInput>(f, 0, act, bias[0], lane, w);
-    for (int l = 1; l + 1 < f.n_mat; ++l) fused_layer<kFHidden>(f, l, act, bias[l], lane, w);
-    fused_layer<kFOutput>(f, f.n_mat - 1, act, bias[f.n_mat - 1], lane, w);
-    const float* zs = reinterpret_cast<const float*>(act);
-    const int valid_rows = f.rows - row0 < kFRows ? f.rows - row0 : kFRows;
-    for (int i = t; i < valid_rows * f.n_out; i += kFThreads) {
-        const int r = i / f.n_out, c = i - r * f.n_out;
-        f.Y[(size_t)row0 * f.n_out + i] = zs[r * 32 + c];
-    }
-}
-
 // Column sums of X (rows x cols, bf16) in row slices: blockIdx.y sums rows
 // [y*row_chunk, (y+1)*row_chunk) into part[y*cols + c] (bias gradients,
 // finished by reduce_partials_kernel in a fixed order).
@@ -535,43 +394,234 @@ struct PairArgs {
     pp_sur_domain dom;
 };
 
-__global__ void __launch_bounds__(256) pair_features_kernel(PairArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.rows) return;
-    const int64_t row = a.row0 + i;
+// Row `row` of a pair list: its features, the log of yield x geometric
+// exposure (R/2r)^2 (expected hits = that x exp(y0)) and the fallback flag
+// (the pair outside the network's domain).
+__device__ __forceinline__ void pair_row(const PairArgs& a, int64_t row, float* f, float& logy,
+                                         bool& fb) {
     const int64_t e = a.n_mod ? row / a.n_mod : row;
     const int64_t j = a.n_mod ? row % a.n_mod : row;
     const pp_cascade c = a.ev[e];
-    float f[PP_SUR_N_FEATURES], r;
+    float r;
     pair_features(c, a.mod[3 * j], a.mod[3 * j + 1], a.mod[3 * j + 2], f, r);
+    logy = c.log_yield + 2.0f * logf(a.dom.module_radius / (2.0f * fmaxf(r, a.dom.module_radius)));
+    fb = !(r >= a.dom.min_dist && r <= a.dom.max_dist && c.energy >= a.dom.min_energy &&
+           c.energy <= a.dom.max_energy);
+}
+
+__global__ void __launch_bounds__(256) pair_features_kernel(PairArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.rows) return;
+    float f[PP_SUR_N_FEATURES], logy;
+    bool fb;
+    pair_row(a, a.row0 + i, f, logy, fb);
     for (int k = 0; k < a.n_in; ++k) {
         const float v = k < PP_SUR_N_FEATURES ? f[k] : 0.0f;
         if (a.xf) a.xf[i * a.n_in + k] = v;
         if (a.xb) a.xb[i * a.n_in + k] = (bf16)v;
     }
-    // expected hits = yield x geometric exposure (R/2r)^2 x exp(y0)
-    if (a.logy) a.logy[i] = c.log_yield + 2.0f * logf(a.dom.module_radius / (2.0f * fmaxf(r, a.dom.module_radius)));
-    if (a.fb)
-        a.fb[i] = !(r >= a.dom.min_dist && r <= a.dom.max_dist && c.energy >= a.dom.min_energy &&
-                    c.energy <= a.dom.max_energy);
+    if (a.logy) a.logy[i] = logy;
+    if (a.fb) a.fb[i] = fb;
 }
 
-// expected hits = exp(y0 + log yield); mixture (weights, shapes, scales in ns)
-__global__ void __launch_bounds__(256) sur_head_kernel(const float* Y, int n_out, int64_t rows,
-                                                       const float* logy, float* hits,
-                                                       float* mix) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    const float* y = Y + i * n_out;
-    hits[i] = __expf(fminf(fmaxf(y[0], kY0Lo), kY0Hi) + logy[i]);
+// Inference head of one row: expected hits = exp(y0 + log yield x exposure);
+// mixture (weights, shapes, scales in ns) into mix[0..8] when mix is set.
+__device__ __forceinline__ void sur_head_row(const float* y, float logy, float* hits, float* mix) {
+    *hits = __expf(fminf(fmaxf(y[0], kY0Lo), kY0Hi) + logy);
     if (!mix) return;
     const float amax = fmaxf(fmaxf(y[1], y[2]), y[3]);
     float e[3], s = 0.0f;
     for (int j = 0; j < 3; ++j) s += (e[j] = __expf(y[1 + j] - amax));
     for (int j = 0; j < 3; ++j) {
-        mix[i * 9 + j] = e[j] / s;
-        mix[i * 9 + 3 + j] = __expf(fminf(fmaxf(y[4 + j], kShapeLo), kShapeHi));
-        mix[i * 9 + 6 + j] = __expf(fminf(fmaxf(y[7 + j], kScaleLo), kScaleHi) + 4.0f);
+        mix[j] = e[j] / s;
+        mix[3 + j] = __expf(fminf(fmaxf(y[4 + j], kShapeLo), kShapeHi));
+        mix[6 + j] = __expf(fminf(fmaxf(y[7 + j], kScaleLo), kScaleHi) + 4.0f);
+    }
+}
+
+__global__ void __launch_bounds__(256) sur_head_kernel(const float* Y, int n_out, int64_t rows,
+                                                       const float* logy, float* hits,
+                                                       float* mix) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    sur_head_row(Y + i * n_out, logy[i], hits + i, mix ? mix + i * 9 : nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: the whole network for a tile of kFRows rows per
+// workgroup, the activations resident in LDS between layers (no HBM round
+// trip). The products are taken transposed, C^T = W . act^T: the weights are
+// MFMA operand A, held in registers and loaded straight from L2 in the packed
+// order of pack_fused_kernel (A row = output column n = lane%32 of the tile,
+// k = 16i + (lane/32)*8 + j; each wave load is 1 KB contiguous), the
+// activations operand B, read from LDS (B column = row m = lane%32). Wave w
+// owns output columns [64w, 64w+64) (two 32-column tiles) of all kFRows rows
+// (four 32-row tiles): each weight fragment feeds four MFMAs and each
+// activation fragment two, and a lane's four consecutive accumulator entries
+// are four consecutive columns of one row, written with one 8-byte LDS store.
+// One activation buffer serves as a layer's input and output (a barrier
+// between the last read and the first write). Each output element is the
+// same 16-term MFMA dot products, K ascending with zero padding to a
+// multiple of 32, and the same bias/SiLU/bf16 epilogue as gemm_kernel.
+// Shapes: hidden width kFH, input and output widths <= 32, <= kFMaxLayers
+// weight matrices.
+// ---------------------------------------------------------------------------
+constexpr int kFH = 256;            // hidden width
+constexpr int kFRows = 128;         // rows per workgroup (four 32-row MFMA tiles)
+constexpr int kFThreads = 256;      // four waves
+constexpr int kFActLd = kFH + 8;    // LDS row stride of the activations (bf16)
+constexpr int kFFrag = kFH / 16;    // 16-deep K fragments of a hidden layer
+constexpr int kFMaxLayers = 8;
+static_assert(kFThreads == kFH, "one thread per bias entry of a layer");
+
+struct FusedArgs {
+    const bf16* X;   // rows x n_in (bf16); unused for pairs
+    int64_t rows;
+    int n_in, n_out, n_mat;
+    const bf16* W[kFMaxLayers];    // layer l packed by pack_fused_kernel
+    const float* b[kFMaxLayers];
+    float* Y;        // rows x n_out (fp32); unused for pairs
+    // pairs: the rows are the (event, module) pairs of pr (ev, mod, n_mod,
+    // dom), whose features are computed in the kernel; pr.fb receives the
+    // fallback flags, and the inference head writes hits and, when set, mix
+    PairArgs pr;
+    float* hits;
+    float* mix;
+};
+
+enum FusedLayer { kFInput = 0, kFHidden = 1, kFOutput = 2 };
+
+// One layer for wave w: products over K in batches of eight weight
+// fragments, barrier (every wave's reads of the layer input done), epilogue
+// into `act` (bf16 SiLU activations, row stride kFActLd) or, for the output
+// layer, fp32 outputs into `act` viewed as float[kFRows][32], barrier. In the
+// output layer (N <= 32, one column tile) wave w takes row tile w.
+template <int MODE>
+__device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act,
+                                            const float* bias_l, int lane, int w) {
+    constexpr int NT = MODE == kFOutput ? 1 : 2;          // 32-column tiles per wave
+    constexpr int MT = MODE == kFOutput ? 1 : 4;          // 32-row tiles per wave
+    constexpr int NF = MODE == kFInput ? 2 : kFFrag;      // K fragments
+    constexpr int NB = NF < 8 ? NF : 8;                   // fragments per load batch
+    const int kl = (lane >> 5) * 8;
+    const int t0 = MODE == kFOutput ? 0 : 2 * w;          // first column tile
+    const int m0 = MODE == kFOutput ? 32 * w : 0;         // first row
+    const bf16x8* wsrc = reinterpret_cast<const bf16x8*>(f.W[l]) + (size_t)t0 * NF * 64 + lane;
+    const bf16* bsrc = act + (m0 + (lane & 31)) * kFActLd + kl;
+    f32x16 acc[NT][MT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = f32x16{};
+#pragma unroll 1
+    for (int i0 = 0; i0 < NF; i0 += NB) {
+        bf16x8 wa[NT][NB];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) wa[nt][i] = wsrc[(nt * NF + i0 + i) * 64];
+        // the batch's loads all in flight before the first product waits on one
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            bf16x8 b[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                b[mt] = *reinterpret_cast<const bf16x8*>(bsrc + 32 * mt * kFActLd + 16 * (i0 + i));
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[nt][mt] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b[mt], acc[nt][mt], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    // acc[nt][mt][4g + j]: column 32(t0 + nt) + 8g + (lane/32)*4 + j, row m0 + 32mt + lane%32
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int n0 = 32 * (t0 + nt) + 8 * g + (lane >> 5) * 4;
+            const float4 b4 = *reinterpret_cast<const float4*>(bias_l + n0);
+            const float bias[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int m = m0 + 32 * mt + (lane & 31);
+                float z[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) z[j] = acc[nt][mt][4 * g + j] + bias[j];
+                if (MODE == kFOutput) {
+                    *reinterpret_cast<float4*>(reinterpret_cast<float*>(act) + m * 32 + n0) =
+                        make_float4(z[0], z[1], z[2], z[3]);
+                } else {
+                    bf16 h[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) h[j] = (bf16)(z[j] * sigmoid(z[j]));
+                    *reinterpret_cast<uint2*>(act + m * kFActLd + n0) =
+                        *reinterpret_cast<const uint2*>(h);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// two workgroups per CU: LDS 76.3 KB each, <= 256 VGPRs per lane.
+// PAIRS: the input tile is computed from (event, module) pairs (pair_row, as
+// pair_features_kernel) and the outputs go through the inference head
+// (sur_head_row, as sur_head_kernel), so that a pair list is evaluated in one
+// launch with the same values as the three-kernel sequence.
+template <bool PAIRS>
+__global__ void __launch_bounds__(kFThreads, 2) mlp_fused_kernel(FusedArgs f) {
+    __shared__ __attribute__((aligned(16))) bf16 act[kFRows * kFActLd];
+    __shared__ __attribute__((aligned(16))) float bias[kFMaxLayers][kFH];
+    __shared__ float logy[PAIRS ? kFRows : 1];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t row0 = (int64_t)blockIdx.x * kFRows;
+    // every layer's biases, zero beyond the layer's width
+    for (int l = 0; l < f.n_mat; ++l)
+        bias[l][t] = t < (l + 1 == f.n_mat ? f.n_out : kFH) ? f.b[l][t] : 0.0f;
+    // input tile, zero beyond n_in (up to 32 columns) and beyond the last row
+    if (PAIRS) {
+        if (t < kFRows) {
+            bf16* a = act + t * kFActLd;
+            const int64_t row = row0 + t;
+            if (row < f.rows) {
+                float feat[PP_SUR_N_FEATURES];
+                bool fb;
+                pair_row(f.pr, row, feat, logy[t], fb);
+                f.pr.fb[row] = fb;
+                for (int k = 0; k < 32; ++k)
+                    a[k] = k < f.n_in ? (bf16)(k < PP_SUR_N_FEATURES ? feat[k] : 0.0f) : (bf16)0.0f;
+            } else {
+                for (int k = 0; k < 32; ++k) a[k] = (bf16)0.0f;
+            }
+        }
+    } else {
+        for (int i = t; i < kFRows * 32; i += kFThreads) {
+            const int r = i >> 5, k = i & 31;
+            const int64_t gr = row0 + r;
+            act[r * kFActLd + k] = (gr < f.rows && k < f.n_in) ? f.X[gr * f.n_in + k]
+                                                               : (bf16)0.0f;
+        }
+    }
+    __syncthreads();
+    fused_layer<kFInput>(f, 0, act, bias[0], lane, w);
+    for (int l = 1; l + 1 < f.n_mat; ++l) fused_layer<kFHidden>(f, l, act, bias[l], lane, w);
+    fused_layer<kFOutput>(f, f.n_mat - 1, act, bias[f.n_mat - 1], lane, w);
+    const float* zs = reinterpret_cast<const float*>(act);
+    const int valid_rows = f.rows - row0 < kFRows ? (int)(f.rows - row0) : kFRows;
+    if (PAIRS) {
+        if (t < valid_rows) {
+            const int64_t row = row0 + t;
+            sur_head_row(zs + t * 32, logy[t], f.hits + row, f.mix ? f.mix + row * 9 : nullptr);
+        }
+    } else {
+        for (int i = t; i < valid_rows * f.n_out; i += kFThreads) {
+            const int r = i / f.n_out, c = i - r * f.n_out;
+            f.Y[row0 * f.n_out + i] = zs[r * 32 + c];
+        }
     }
 }
 
@@ -659,19 +709,28 @@ bool fused_ok(const pp_mlp* M) {
     return true;
 }
 
-int forward_fused(pp_mlp* M, int rows) {
-    if (!M->pk_ok) {
-        for (int l = 0; l < M->n_mat; ++l) {
-            const int K = M->dims[l], N = M->dims[l + 1];
-            const int nf = (K + 31) / 32 * 2;
-            const int64_t chunks = (M->pk_off[l + 1] - M->pk_off[l]) / 8;
-            hipLaunchKernelGGL(pack_fused_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0,
-                               M->stream, M->pb + M->w_off[l], K, N, nf, chunks,
-                               M->wpk + M->pk_off[l]);
-            PP_HIP(hipGetLastError());
-        }
-        M->pk_ok = true;
+// the fused kernel's packed copy of the bf16 weights, refreshed after a change
+int ensure_packed(pp_mlp* M) {
+    if (M->pk_ok) return PP_OK;
+    for (int l = 0; l < M->n_mat; ++l) {
+        const int K = M->dims[l], N = M->dims[l + 1];
+        const int nf = (K + 31) / 32 * 2;
+        const int64_t chunks = (M->pk_off[l + 1] - M->pk_off[l]) / 8;
+        hipLaunchKernelGGL(pack_fused_kernel, dim3(blocks_for(chunks, 256)), dim3(256), 0,
+                           M->stream, M->pb + M->w_off[l], K, N, nf, chunks, M->wpk + M->pk_off[l]);
+        PP_HIP(hipGetLastError());
     }
+    M->pk_ok = true;
+    return PP_OK;
+}
+
+// The fused kernel over `rows` rows: the bf16 input in H[0] to the fp32
+// outputs in Y, or, with pairs set, the pair list's features to the head's
+// outputs (hits, mix, pairs->fb).
+int forward_fused(pp_mlp* M, int64_t rows, const PairArgs* pairs = nullptr, float* hits = nullptr,
+                  float* mix = nullptr) {
+    if (int rc = ensure_packed(M)) return rc;
+    if (rows <= 0) return PP_OK;
     FusedArgs f{};
     f.X = M->H[0];
     f.rows = rows;
@@ -683,8 +742,15 @@ int forward_fused(pp_mlp* M, int rows) {
         f.b[l] = M->p + M->b_off[l];
     }
     f.Y = M->Y;
-    hipLaunchKernelGGL(mlp_fused_kernel, dim3(blocks_for(rows, kFRows)), dim3(kFThreads), 0,
-                       M->stream, f);
+    const dim3 grid(blocks_for(rows, kFRows));
+    if (pairs) {
+        f.pr = *pairs;
+        f.hits = hits;
+        f.mix = mix;
+        hipLaunchKernelGGL(mlp_fused_kernel<true>, grid, dim3(kFThreads), 0, M->stream, f);
+    } else {
+        hipLaunchKernelGGL(mlp_fused_kernel<false>, grid, dim3(kFThreads), 0, M->stream, f);
+    }
     PP_HIP(hipGetLastError());
     return PP_OK;
 }
@@ -1268,7 +1334,19 @@ int pp_sur_eval_pairs(pp_mlp* M, const pp_cascade* events, int n_events, const f
     PP_HIP(hipMemcpyAsync(M->d_mod, module_xyz, (size_t)n_modules * 12, hipMemcpyHostToDevice,
                           M->stream));
     PP_HIP(hipEventRecord(M->ev0, M->stream));
-    for (int64_t r0 = 0; r0 < n; r0 += M->max_batch) {
+    if (M->fused && M->wpk) {
+        // one launch over every pair: features, network and head in one kernel
+        PairArgs a{};
+        a.ev = M->d_ev;
+        a.mod = M->d_mod;
+        a.n_mod = n_modules;
+        a.rows = n;
+        a.n_in = M->dims[0];
+        a.fb = M->fb;
+        a.dom = *dom;
+        if (int rc = forward_fused(M, n, &a, M->hits, mixture ? M->mix : nullptr)) return rc;
+    }
+    for (int64_t r0 = 0; !(M->fused && M->wpk) && r0 < n; r0 += M->max_batch) {
         const int rows = (int)std::min<int64_t>(M->max_batch, n - r0);
         PairArgs a{};
         a.ev = M->d_ev;
