@@ -1355,7 +1355,7 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
                  float* dist_travelled, int64_t* module_ix, int32_t* n_steps,
                  uint64_t* photon_id, int32_t* source_ix) {
     if (!ctx) return fail(PP_ERR_INVALID, "pp_copy_hits: NULL ctx");
-    if (first < 0 || n < 0 || first + n > ctx->n_hits)
+    if (first < 0 || n < 0 || first > ctx->n_hits || n > ctx->n_hits - first)
         return fail(PP_ERR_INVALID, "pp_copy_hits: range outside [0, n_hits)");
     if (n == 0) return PP_OK;
     if (int rc = set_dev(ctx)) return rc;

@@ -171,6 +171,10 @@ def test_hit_buffer_overflow_raises(P, O, gpu_lib):
         pos = np.zeros((8, 3), np.float32)
         assert lib.pp_copy_hits(small.handle, 0, 8, P._lib.fptr(pos), None, None, None, None, 0,
                                 None, None, None, P._lib.fptr(pid), None) == 0
+        # ranges beyond the stored records are refused, also where first + n overflows
+        for first, n in ((0, 9), (8, 1), (9, 0), (1, 2**63 - 1), (-1, 1)):
+            assert lib.pp_copy_hits(small.handle, first, n, P._lib.fptr(pos), None, None, None,
+                                    None, 0, None, None, None, None, None) == P._lib.PP_ERR_INVALID
         r, _ = oracle_run(P, O, wl)
         idx = {int(p): i for i, p in enumerate(r["photon_id"])}
         hits = [idx.get(int(p)) for p in pid]
