@@ -73,6 +73,25 @@ __host__ __device__ __forceinline__ float rcp_clamped(float x) {
 #endif
 }
 
+// a * b for 0 <= a, b and a * b < 2^24 (cell indices: a grid has fewer
+// cells than fit in LDS): the full-rate 24-bit multiply on the device
+__host__ __device__ __forceinline__ int mul24(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
+// x, opaque to the optimiser: a value computed from it stays where it is
+// written instead of being hoisted out of a rarely taken branch
+__host__ __device__ __forceinline__ float pinned(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+v"(x));
+#endif
+    return x;
+}
+
 __host__ __device__ __forceinline__ int clampi(int v, int hi) {
     return v < 0 ? 0 : (v > hi ? hi : v);
 }
@@ -174,11 +193,14 @@ struct GridSearch {
         // first, the next index (wrapped into another row, or clamped into the
         // table) names a cell off the segment, whose targets are tested
         // exactly and cannot change the smallest hit index.
-        // the cell as the byte offset of its record pair (32 bytes a cell)
-        const int dcx = 32 * sx, dcy = 32 * sy * g.nx, dcz = 32 * sz * g.nx * g.ny;
+        // the cell as the byte offset of its record pair (32 bytes a cell);
+        // its steps are selects between uniform constants, not products
+        const int row = 32 * g.nx, layer = 32 * g.nx * g.ny;
+        const int dcx = sx > 0 ? 32 : -32, dcy = sy > 0 ? row : -row;
+        const int dcz = COLS ? 0 : (sz > 0 ? layer : -layer);
         const int c_max = 32 * (g.n_cells - 1);
         const char* const rec = reinterpret_cast<const char*>(cells);
-        int c = 32 * ((cz * g.ny + cy) * g.nx + cx);
+        int c = 32 * (mul24(mul24(cz, g.ny) + cy, g.nx) + cx);
         const int max_visits = g.nx + g.ny + g.nz;
         for (int v = 0; v < max_visits; ++v) {
             // this cell's record; the walk's next step is computed while it loads
@@ -192,7 +214,7 @@ struct GridSearch {
             const bool az = !ax && !ay;
             const float tn = ax ? tmx : (ay ? tmy : tmz);
             const bool last = tn >= t_out;
-            const int cn = min(max(c + (ax ? dcx : (ay ? dcy : dcz)), 0), c_max);
+            const int cn = min(max(c + (ax ? dcx : (COLS || ay ? dcy : dcz)), 0), c_max);
             if (COUNT) w.cells += 1;
             // slab test of the segment [0, step] against the box of this
             // cell's targets (an empty cell holds a box at +inf, which fails
@@ -220,7 +242,8 @@ struct GridSearch {
             if (pass) {
                 if (COLS) b_out = fminf(b_out, step);
                 const uint32_t b = float_bits(lo.w), cnt = float_bits(hi.w);
-                const float exc = dir_excess(dir);  // rarely needed: not kept across the walk
+                // rarely needed: computed here, not hoisted in front of the walk
+                const float exc = dir_excess(make3(pinned(dir.x), pinned(dir.y), pinned(dir.z)));
                 // a target is hit inside the box, so within its radius of the
                 // segment's z range there: targets farther in z are skipped
                 // (columns hold a whole string; a ray crossing it meets one or
