@@ -229,6 +229,60 @@ def test_statistical_parity_cascade(P, O, dev):
     del mods
 
 
+@pytest.mark.parametrize("name", ["cascade_16_strings", "uneven_batch_1000"])
+def test_full_size_workloads(P, name):
+    """The benchmark's workloads at their full sizes (1e8 photons of one
+    cascade; 256 cascades and tracks, ~3.7e8 photons, 1000 modules), checked
+    through properties that hold at any size: every photon accounted for;
+    the same hits, bit for bit, under two launch schedules (work split and
+    chunking differ, the per-photon streams do not); and every hit physical:
+    a module of the list, 1..n_steps steps, on its sphere, and arriving
+    dist / c_grp(wavelength) after its emission at the source's time (plus
+    the emission point's offset along a cascade or track, >= 0)."""
+    wl = getattr(P.workloads, name)()
+    runs = []
+    for block, chunk in ((256, 256), (512, 2048)):
+        cfg = P._lib.pp_config()
+        cfg.block_size = block
+        cfg.chunk_photons = chunk
+        cfg.deterministic_order = 1
+        _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
+        with dev:
+            runs.append(gpu_run(P, dev, wl))
+    (g, n1), (g2, n2) = runs
+    expected = sum(int(np.float32(s.photons) * np.float32(1.0)) for s in wl.sources)
+    assert n1 == n2 == expected
+    n_hits = len(g["wavelength"])
+    print(f"{name}: {expected} photons, {n_hits} hits")
+    assert n_hits > 10_000
+    for k in g:
+        assert g[k].tobytes() == g2[k].tobytes(), k
+    mid = g["module_id"]
+    assert mid.min() >= 1 and mid.max() <= len(wl.targets)
+    assert g["n_steps"].min() >= 1 and g["n_steps"].max() <= wl.n_steps
+    centre = np.array([t.shape.position for t in wl.targets], np.float64)[mid - 1]
+    radius = np.array([t.shape.radius for t in wl.targets], np.float64)[mid - 1]
+    # on the sphere, up to the reference's discriminant r^2 - |q|^2 + eps a^2
+    # (eps = |dir|^2 - 1 of the Float32 direction), which puts the crossing at
+    # |p - c|^2 = r^2 + eps d^2 for a crossing d along the step (d <= the
+    # distance travelled), plus Float32 round-off of positions ~100 m out
+    p2 = np.sum((g["position"].astype(np.float64) - centre) ** 2, axis=1)
+    d = g["direction"].astype(np.float64)
+    eps = np.abs(np.sum(d * d, axis=1) - 1.0)
+    excess = np.abs(p2 - radius ** 2)
+    bound = 1.05 * eps * g["dist_travelled"].astype(np.float64) ** 2 + 1e-3
+    off = np.sqrt(p2) - radius
+    print(f"distance from the sphere: median {np.median(np.abs(off)):.2e} m, max "
+          f"{np.abs(off).max():.2e} m; max excess / bound {np.max(excess / bound):.3f}")
+    assert np.median(np.abs(off)) < 1e-3
+    assert np.all(excess <= bound)
+    t_src = np.array([s.time for s in wl.sources], np.float64)[g["source_ix"]]
+    lag = g["time"] - g["dist_travelled"] / wl.medium.group_velocity(g["wavelength"]) - t_src
+    assert np.all(lag >= -(1e-3 + 1e-5 * g["time"])), lag.min()
+    max_offset = max(float(getattr(s, "length", 0.0) or 0.0) for s in wl.sources) + 100.0
+    assert np.all(lag <= max_offset / 0.299792458 + 1e-3 + 1e-5 * g["time"]), lag.max()
+
+
 @pytest.mark.parametrize("cell,cell_z", [(0.0, 0.0), (4.0, 0.0), (30.0, 0.0), (0.0, -1.0),
                                          (20.0, 60.0)])
 def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell, cell_z):
