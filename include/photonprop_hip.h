@@ -319,12 +319,14 @@ int pp_pom_pmt_directions(const pp_pom_model* model, float* dirs16x3);
 /* pmt_id_out[i] = 1..16 for a detected hit, 0 otherwise. rel_pos3: n x 3;
  * wavelength (nm), photon_id and source_ix: n each (the photon's stream);
  * wl_grid, acc1, acc2: model->n_wl each (acceptances of groups 1 and 2).
- * *n_detected (may be NULL) counts the detected hits. Runs on `device`. */
+ * *n_detected (may be NULL) counts the detected hits; *kernel_ms (may be
+ * NULL) receives the device time of the detection kernel alone (HIP events;
+ * the host copies around it excluded). Runs on `device`. */
 int pp_pom_pmt_hits(int device, const pp_pom_model* model, const float* wl_grid,
                     const float* acc1, const float* acc2, int64_t n, const float* rel_pos3,
                     const float* wavelength, const uint64_t* photon_id,
                     const int32_t* source_ix, uint64_t seed, int32_t* pmt_id_out,
-                    int64_t* n_detected);
+                    int64_t* n_detected, float* kernel_ms);
 
 /* Autotuner. The reference fixes its launch shape from the occupancy API
  * (src/photon_prop_cuda.jl:914-920); this library searches it instead.

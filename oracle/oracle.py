@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/libpp_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker. The product path
+cpu_baseline leg of bench.py, as the reference values the GPU results are
+compared against. The product path
 (photonprop, libphotonprop_hip.so) never imports it.
 """
 from __future__ import annotations

@@ -1,8 +1,8 @@
 /*
  * pp_mlp_oracle.c - CPU oracle of the surrogate MLP (include/photonprop_surrogate.h).
  *
- * TEST INFRASTRUCTURE ONLY, like pp_oracle.c: the checker of the GPU's
- * forward pass, loss head, gradients and Adam step. Everything is computed in
+ * TEST INFRASTRUCTURE ONLY, like pp_oracle.c: the reference values for the
+ * GPU's forward pass, loss head, gradients and Adam step. Everything is computed in
  * double. With quant = 1 the values are rounded to bf16 (round to nearest
  * even) at exactly the points where the GPU kernels hold bf16: the input
  * features, the weights, each hidden activation, the output gradient and each

@@ -4,6 +4,7 @@
 // kernel arguments (SGPRs); the acceptance tables sit in LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -60,52 +61,67 @@ __device__ __forceinline__ float interp0(const float* x, const float* y, int n, 
     return y[lo] + t * (y[hi] - y[lo]);
 }
 
+// Grid-stride over the hits (a few workgroups per CU), so that the detected
+// count takes one atomic per workgroup: a wave ballot per pass, the waves'
+// sums in LDS at the end.
 __global__ void __launch_bounds__(256) pom_pmt_kernel(PmtArgs a) {
     __shared__ float tab[3 * kMaxWl];
+    __shared__ unsigned long long wave_det[4];
     for (int i = threadIdx.x; i < 3 * a.n_wl; i += blockDim.x) tab[i] = a.tab[i];
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool det = false;
-    if (i < a.n) {
-        const float px = a.rel_pos[3 * i], py = a.rel_pos[3 * i + 1], pz = a.rel_pos[3 * i + 2];
-        const float wl = a.wl[i];
-        const float acc[2] = {interp0(tab, tab + a.n_wl, a.n_wl, wl),
-                              interp0(tab, tab + 2 * a.n_wl, a.n_wl, wl)};
-        float p[kPmts], sum = 0.0f;
+    unsigned long long n_det = 0;  // wave-uniform
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.n; base += stride) {
+        const int64_t i = base + threadIdx.x;
+        bool det = false;
+        if (i < a.n) {
+            const float px = a.rel_pos[3 * i], py = a.rel_pos[3 * i + 1], pz = a.rel_pos[3 * i + 2];
+            const float wl = a.wl[i];
+            const float acc[2] = {interp0(tab, tab + a.n_wl, a.n_wl, wl),
+                                  interp0(tab, tab + 2 * a.n_wl, a.n_wl, wl)};
+            float p[kPmts], sum = 0.0f;
 #pragma unroll
-        for (int j = 0; j < kPmts; ++j) {
-            const int g = (j >> 2) & 1;
-            const float c = fminf(fmaxf(px * a.dir[j][0] + py * a.dir[j][1] + pz * a.dir[j][2], -1.0f),
-                                  1.0f);
-            const float pt = acosf(c);
-            const float st = sinf(pt);
-            // Rayleigh pdf over the density 0.5 sin(pt) of a uniform flux
-            const float pdf = pt * a.inv_s2[g] * __expf(-0.5f * pt * pt * a.inv_s2[g]);
-            p[j] = st != 0.0f ? pdf / (0.5f * st) * acc[g] * 0.125f : 0.0f;
-            sum += p[j];
-        }
-        const uint64_t id = a.pid[i];
-        const uint4 r = pp::philox(make_uint4(PP_CTR_PMT, (uint32_t)id, (uint32_t)(id >> 32),
-                                              (uint32_t)a.src[i]),
-                                   a.k0, a.k1);
-        int pmt = 0;
-        if (pp::u01(r.x) < sum) {
-            // categorical draw: first j whose cumulative weight exceeds u * sum
-            const float u = pp::u01(r.y) * sum;
-            float cum = 0.0f;
             for (int j = 0; j < kPmts; ++j) {
-                cum += p[j];
-                if (p[j] > 0.0f) {
-                    pmt = j + 1;  // the last PMT with weight if rounding leaves u >= cum
-                    if (u < cum) break;
-                }
+                const int g = (j >> 2) & 1;
+                const float c = fminf(fmaxf(px * a.dir[j][0] + py * a.dir[j][1] + pz * a.dir[j][2],
+                                            -1.0f), 1.0f);
+                const float pt = acosf(c);
+                // sin(acos c) = sqrt((1 - c)(1 + c)), exact factors (no sinf)
+                const float st = __builtin_amdgcn_sqrtf((1.0f - c) * (1.0f + c));
+                // Rayleigh pdf over the density 0.5 sin(pt) of a uniform flux
+                const float pdf = pt * a.inv_s2[g] * __expf(-0.5f * pt * pt * a.inv_s2[g]);
+                p[j] = st != 0.0f ? pdf * (2.0f * __builtin_amdgcn_rcpf(st)) * acc[g] * 0.125f : 0.0f;
+                sum += p[j];
             }
-            det = true;
+            const uint64_t id = a.pid[i];
+            const uint4 r = pp::philox(make_uint4(PP_CTR_PMT, (uint32_t)id, (uint32_t)(id >> 32),
+                                                  (uint32_t)a.src[i]),
+                                       a.k0, a.k1);
+            int pmt = 0;
+            if (pp::u01(r.x) < sum) {
+                // categorical draw: first j whose cumulative weight exceeds u * sum
+                const float u = pp::u01(r.y) * sum;
+                float cum = 0.0f;
+                for (int j = 0; j < kPmts; ++j) {
+                    cum += p[j];
+                    if (p[j] > 0.0f) {
+                        pmt = j + 1;  // the last PMT with weight if rounding leaves u >= cum
+                        if (u < cum) break;
+                    }
+                }
+                det = true;
+            }
+            a.out[i] = pmt;
         }
-        a.out[i] = pmt;
+        n_det += (unsigned long long)__popcll(__ballot(det));
     }
-    const uint64_t m = __ballot(det);
-    if (__lane_id() == 0 && m) atomicAdd(a.n_det, (unsigned long long)__popcll(m));
+    if (__lane_id() == 0) wave_det[threadIdx.x >> 6] = n_det;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wave_det[w];
+        if (s) atomicAdd(a.n_det, s);
+    }
 }
 
 // PMT (theta, phi) of the POM, src/pom.jl:108-128
@@ -163,10 +179,11 @@ int pp_pom_pmt_directions(const pp_pom_model* m, float* out) {
 int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, const float* acc1,
                     const float* acc2, int64_t n, const float* rel_pos3, const float* wavelength,
                     const uint64_t* photon_id, const int32_t* source_ix, uint64_t seed,
-                    int32_t* pmt_id_out, int64_t* n_detected) {
+                    int32_t* pmt_id_out, int64_t* n_detected, float* kernel_ms) {
     if (int rc = validate(m)) return rc;
     if (n < 0 || n > ((int64_t)1 << 31)) return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: n out of range");
     if (n_detected) *n_detected = 0;
+    if (kernel_ms) *kernel_ms = 0.0f;
     if (n == 0) return PP_OK;
     if (!wl_grid || !acc1 || !acc2 || !rel_pos3 || !wavelength || !photon_id || !source_ix ||
         !pmt_id_out)
@@ -228,8 +245,27 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, con
         a.src = (const int32_t*)(d + off_src);
         a.out = (int32_t*)(d + off_out);
         a.n_det = (unsigned long long*)(d + off_cnt);
-        hipLaunchKernelGGL(pom_pmt_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (kernel_ms) {
+            chk(hipEventCreate(&e0), "event");
+            chk(hipEventCreate(&e1), "event");
+            if (rc == PP_OK) chk(hipEventRecord(e0, 0), "event record");
+        }
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, (int64_t)cus * 8);
+        if (rc == PP_OK)
+            hipLaunchKernelGGL(pom_pmt_kernel, dim3((unsigned)blocks), dim3(256), 0, 0, a);
         chk(hipGetLastError(), "pom_pmt_kernel launch");
+        if (kernel_ms && rc == PP_OK) {
+            chk(hipEventRecord(e1, 0), "event record");
+            chk(hipEventSynchronize(e1), "event sync");
+            if (rc == PP_OK) chk(hipEventElapsedTime(kernel_ms, e0, e1), "event time");
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
         chk(hipMemcpy(pmt_id_out, d + off_out, (size_t)n * 4, hipMemcpyDeviceToHost), "copy ids");
         unsigned long long cnt = 0;
         chk(hipMemcpy(&cnt, d + off_cnt, 8, hipMemcpyDeviceToHost), "copy count");

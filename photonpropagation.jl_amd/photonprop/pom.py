@@ -118,7 +118,7 @@ def check_pmt_hit(hit_positions, hit_directions, hit_wavelengths, target: POM, o
                     orientation, photon_id, source_ix, seed, device)
 
 
-def _pmt_ids(rel_pos, wl, acc, orientation, photon_id, source_ix, seed, device):
+def _pmt_ids(rel_pos, wl, acc, orientation, photon_id, source_ix, seed, device, timing=None):
     n = len(rel_pos)
     rp = np.ascontiguousarray(rel_pos, np.float32)
     w = np.ascontiguousarray(wl, np.float32)
@@ -129,10 +129,14 @@ def _pmt_ids(rel_pos, wl, acc, orientation, photon_id, source_ix, seed, device):
     a2 = np.ascontiguousarray(acc.acc_2, np.float32)
     out = np.zeros(n, np.int32)
     nd = ctypes.c_int64()
+    ms = ctypes.c_float()
     m = _model(acc, orientation)
     _lib.check(_lib.load_library().pp_pom_pmt_hits(
         device, ctypes.byref(m), _p(grid), _p(a1), _p(a2), n, _p(rp), _p(w), _p(pid), _p(src),
-        ctypes.c_uint64(seed), _p(out), ctypes.byref(nd)), "pp_pom_pmt_hits")
+        ctypes.c_uint64(seed), _p(out), ctypes.byref(nd), ctypes.byref(ms)), "pp_pom_pmt_hits")
+    if timing is not None:
+        timing["kernel_ms"] = float(ms.value)
+        timing["n_detected"] = int(nd.value)
     return out
 
 
