@@ -225,22 +225,19 @@ struct GridSearch {
             // filter and the exact sphere test below then settle.
             const float ax0 = fmaf(lo.x, ix_, -px_), ax1 = fmaf(hi.x, ix_, -px_);
             const float ay0 = fmaf(lo.y, iy_, -py_), ay1 = fmaf(hi.y, iy_, -py_);
+            // the end of the segment as a compare of its own (a minimum with
+            // step would canonicalise step in every iteration)
             float b_in, b_out;
-            bool pass;
             if (COLS) {
-                // the end of the segment as a compare of its own (a minimum
-                // with step would canonicalise step in every iteration)
                 b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), 0.0f);
                 b_out = fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1));
-                pass = b_in <= b_out && b_in <= step;
             } else {
                 const float az0 = fmaf(lo.z, iz_, -pz_), az1 = fmaf(hi.z, iz_, -pz_);
                 b_in = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), 0.0f));
-                b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), step));
-                pass = b_in <= b_out;
+                b_out = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1));
             }
-            if (pass) {
-                if (COLS) b_out = fminf(b_out, step);
+            if (b_in <= b_out && b_in <= step) {
+                b_out = fminf(b_out, step);
                 const uint32_t b = float_bits(lo.w), cnt = float_bits(hi.w);
                 // rarely needed: computed here, not hoisted in front of the walk
                 const float exc = dir_excess(make3(pinned(dir.x), pinned(dir.y), pinned(dir.z)));
