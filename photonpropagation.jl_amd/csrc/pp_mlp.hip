@@ -522,6 +522,9 @@ __device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act
             for (int i = 0; i < NB; ++i) wa[nt][i] = wsrc[(nt * NF + i0 + i) * 64];
         // the batch's loads all in flight before the first product waits on one
         __builtin_amdgcn_sched_barrier(0);
+        // issue priority to the waves feeding the matrix cores, so that the
+        // other workgroup's epilogue fills the vector slots they leave
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             bf16x8 b[MT];
@@ -535,6 +538,7 @@ __device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act
                     acc[nt][mt] =
                         __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b[mt], acc[nt][mt], 0, 0, 0);
         }
+        __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
     // acc[nt][mt][4g + j]: column 32(t0 + nt) + 8g + (lane/32)*4 + j, row m0 + 32mt + lane%32
