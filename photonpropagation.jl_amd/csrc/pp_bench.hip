@@ -9,9 +9,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <string>
 
 #include "../../include/photonprop_hip.h"
+
+// the library's per-thread error message (pp_last_error)
+void pp_set_error_message(const char* msg);
 
 namespace {
 
@@ -63,24 +65,31 @@ __global__ void __launch_bounds__(256) mb_mad64(float* out, uint32_t m) {
     out[blockIdx.x * blockDim.x + threadIdx.x] = __uint_as_float(s);
 }
 
-thread_local std::string g_mb_err;
-
 }  // namespace
 
 extern "C" int pp_microbench(int device, int which, double* rate) {
     if (!rate || which < 0 || which > 2) {
-        g_mb_err = "pp_microbench: bad arguments";
+        pp_set_error_message("pp_microbench: bad arguments");
         return PP_ERR_INVALID;
     }
-    if (hipSetDevice(device) != hipSuccess) return PP_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) {
+        pp_set_error_message("pp_microbench: hipSetDevice failed");
+        return PP_ERR_HIP;
+    }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PP_ERR_HIP;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        pp_set_error_message("pp_microbench: hipGetDeviceProperties failed");
+        return PP_ERR_HIP;
+    }
     // 8 waves of 64 per SIMD, 4 SIMDs per CU
     const int block = 256;
     const int blocks = prop.multiProcessorCount * 8;
     const size_t n = (size_t)blocks * block;
     float* out = nullptr;
-    if (hipMalloc(&out, n * sizeof(float)) != hipSuccess) return PP_ERR_HIP;
+    if (hipMalloc(&out, n * sizeof(float)) != hipSuccess) {
+        pp_set_error_message("pp_microbench: device allocation failed");
+        return PP_ERR_HIP;
+    }
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -103,7 +112,10 @@ extern "C" int pp_microbench(int device, int which, double* rate) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(out);
-    if (rc != PP_OK) return rc;
+    if (rc != PP_OK) {
+        pp_set_error_message("pp_microbench: kernel failed");
+        return rc;
+    }
     const double ops = (double)n * kIters * kChains * (which == PP_MB_FMA_F32 ? 2.0 : 1.0);
     *rate = ops / (best * 1e-3);
     return PP_OK;
