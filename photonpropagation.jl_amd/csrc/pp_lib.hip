@@ -1187,11 +1187,12 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     // photon count of src/photon_prop_cuda.jl:456 (Float32 product)
     std::vector<uint64_t> nph((size_t)n_sources);
     for (int i = 0; i < n_sources; ++i) {
-        const int64_t n = (int64_t)std::floor((float)sources[i].photons * photon_scaling);
-        nph[(size_t)i] = n > 0 ? (uint64_t)n : 0;
-        if (nph[(size_t)i] >= kMaxPhotonsPerSource)
+        // the limit is checked on the float product, before it is converted
+        const float prod = std::floor((float)sources[i].photons * photon_scaling);
+        if (!(prod < (float)kMaxPhotonsPerSource))
             return fail(PP_ERR_INVALID, "pp_propagate: source " + std::to_string(i) +
                                             " has 2^40 or more photons after scaling");
+        nph[(size_t)i] = prod > 0.0f ? (uint64_t)prod : 0;
     }
     ctx->n_launch = 0;
     const int mode = ctx->cfg.batch_mode == PP_BATCH_AUTO ? PP_BATCH_PERSISTENT : ctx->cfg.batch_mode;

@@ -185,6 +185,23 @@ def test_hit_buffer_overflow_raises(P, O, gpu_lib):
         del ctypes
 
 
+def test_photon_count_limit(P, dev):
+    """A source of 2^40 or more photons after the Float32 scaling product (the
+    photon id field's width) is refused, also where the product exceeds the
+    int64 range, and the context stays usable."""
+    medium = P.CascadiaMediumProperties()
+    targets = P.make_detector_grid(2, 2, 15.0, 4, 8.0, radius=0.4)
+    for photons, scaling in ((2**40, 1.0), (10**12, 1e10), (2**62, 1.0)):
+        src = P.PointlikeIsotropicEmitter(np.float32([0, 0, 0]), 0.0, photons)
+        wl = P.workloads.Workload("limit", [src], targets, medium, P.Monochromatic(400.0), 1)
+        with pytest.raises(P._lib.PhotonPropLibraryError, match="2\\^40"):
+            gpu_run(P, dev, wl, photon_scaling=scaling)
+    src = P.PointlikeIsotropicEmitter(np.float32([0, 0, 0]), 0.0, 1000)
+    wl = P.workloads.Workload("limit", [src], targets, medium, P.Monochromatic(400.0), 1)
+    _, n = gpu_run(P, dev, wl)
+    assert n == 1000
+
+
 def test_zero_photons_and_zero_steps(P, dev):
     wl = P.workloads.readme_point_source(photons=0)
     g, n = gpu_run(P, dev, wl)
