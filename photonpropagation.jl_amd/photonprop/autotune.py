@@ -81,8 +81,13 @@ def autotune(dev: DeviceHitBuffer, sources, targets, medium, spectrum, seed: int
     path = cache_path or default_cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
-        with open(path) as f:
-            cache = json.load(f)
+        try:
+            with open(path) as f:
+                cache = json.load(f)
+            if not isinstance(cache, dict):
+                cache = {}
+        except (OSError, ValueError):
+            cache = {}  # unreadable or corrupt: tune again and rewrite it
     if use_cache and key in cache:
         ent = cache[key]
         cfg = _lib.pp_config()
@@ -100,9 +105,12 @@ def autotune(dev: DeviceHitBuffer, sources, targets, medium, spectrum, seed: int
            "sample_photons": float(sample_photons), "arch": arch}
     if use_cache:
         cache[key] = ent
-        os.makedirs(os.path.dirname(path), exist_ok=True)
-        tmp = path + ".tmp"
-        with open(tmp, "w") as f:
-            json.dump(cache, f, indent=1, sort_keys=True)
-        os.replace(tmp, path)
+        tmp = f"{path}.{os.getpid()}.tmp"  # one writer per process; the rename is atomic
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(tmp, "w") as f:
+                json.dump(cache, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+        except OSError:
+            pass  # an unwritable cache only costs a tuning run next time
     return {**ent, "cached": False, "key": key}

@@ -428,6 +428,19 @@ def test_autotune_picks_a_candidate_and_keeps_results(P, tmp_path):
         got = P._lib.pp_config()
         P._lib.load_library().pp_get_config(d2.handle, __import__("ctypes").byref(got))
         assert P.autotune.config_to_dict(got) == c
+    # a corrupt cache is tuned again and rewritten; an unwritable one only skips caching
+    with open(cache, "w") as f:
+        f.write("{not json")
+    blocked = tmp_path / "file"
+    blocked.write_text("")
+    for path, cached_after in ((cache, True), (str(blocked / "sub" / "tune.json"), False)):
+        _, d3 = P.make_hit_buffers(capacity=1 << 21, config=cfg)
+        with d3:
+            r3 = P.autotune.autotune(d3, wl.sources, wl.targets, wl.medium, wl.spectrum, wl.seed,
+                                     n_steps=wl.n_steps, sample_photons=2e6, cache_path=path)
+            assert not r3["cached"]
+        import os
+        assert os.path.exists(path) == cached_after
 
 
 def test_ice_medium_photon_parity(P, O, dev):
