@@ -655,8 +655,12 @@ int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
         xyz[3 * i + 2] = c->h_targets[i].z;
     }
     pp::GridTables t;
+    // the occupancy-friendly budget first; for many targets (their records
+    // alone may fill it) the whole LDS, at fewer workgroups per CU
     if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
-                               c->spec_n, kGridLdsBudget, t))
+                               c->spec_n, kGridLdsBudget, t) &&
+        !pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
+                               c->spec_n, kMaxLdsBytes, t))
         return fail(PP_ERR_INVALID, "grid: no cell size fits the LDS budget");
     if (t.cells.size() > c->cell_cap) {
         if (c->d_cells) PP_HIP(hipFree(c->d_cells));
@@ -1147,17 +1151,22 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
         return fail(PP_ERR_INVALID,
                     "pp_propagate: the grid search takes spherical targets only (the list holds "
                     "rectangles or discs)");
-    if (search == PP_SEARCH_AUTO)
+    const bool automatic = search == PP_SEARCH_AUTO;
+    if (automatic)
         search = ctx->n_targets > 8 && !ctx->planar ? PP_SEARCH_GRID : PP_SEARCH_BRUTE;
     if (ctx->n_targets == 0) search = PP_SEARCH_BRUTE;
-    const bool grid = search == PP_SEARCH_GRID;
-    const int skind = grid ? kSearchGrid : ctx->planar ? kSearchMixed : kSearchBrute;
-    if (grid) {
+    if (search == PP_SEARCH_GRID) {
         const float want = ctx->cfg.grid_cell, want_z = ctx->cfg.grid_cell_z;
         if (ctx->grid_cell_built != want || ctx->grid_cell_z_built != want_z ||
-            ctx->grid_cell_built < -0.5f)
-            if (int rc = build_grid(ctx, want, want_z)) return rc;
+            ctx->grid_cell_built < -0.5f) {
+            if (int rc = build_grid(ctx, want, want_z)) {
+                if (!automatic) return rc;
+                search = PP_SEARCH_BRUTE;  // no grid fits: the every-target scan
+            }
+        }
     }
+    const bool grid = search == PP_SEARCH_GRID;
+    const int skind = grid ? kSearchGrid : ctx->planar ? kSearchMixed : kSearchBrute;
     const size_t lds = lds_bytes(ctx->planar ? 2 * ctx->n_targets : ctx->n_targets,
                                  grid ? ctx->grid.n_cells : 0,
                                  grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);

@@ -332,6 +332,52 @@ def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell, cell_z):
         assert gb[k].tobytes() == gg[k].tobytes(), k
 
 
+def test_lattice_of_4096_spheres(P, O, gpu_lib):
+    """A 16 x 16 x 16 lattice of spheres (4096 targets: their records alone
+    fill the grid's usual 64 KB of LDS, so the grid takes the whole LDS)
+    around a point source: the automatic grid (cubic cells, a 3-D walk) gives
+    the brute-force scan's hits bit for bit, both agree photon by photon with
+    the oracle, and the automatic search picks the grid."""
+    import ctypes
+    import itertools
+    medium = P.CascadiaMediumProperties()
+    xs = np.arange(16) * 4.0 - 30.0
+    targets = [P.spherical_target((x, y, z), 0.5, i + 1)
+               for i, (x, y, z) in enumerate(itertools.product(xs, xs, xs))]
+    src = P.PointlikeIsotropicEmitter(np.float32([0.3, -0.7, 1.1]), 0.0, 100_000)
+    wl = P.workloads.Workload("lattice", [src], targets, medium, P.Monochromatic(420.0), 21)
+    out = {}
+    for search in (1, 2):
+        cfg = P._lib.pp_config()
+        cfg.search = search
+        cfg.deterministic_order = 1
+        _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
+        with d:
+            out[search] = gpu_run(P, d, wl)
+            info, h, ni = ctypes.c_int32(), ctypes.c_float(), ctypes.c_int32()
+            dims = (ctypes.c_int32 * 3)()
+            gpu_lib.pp_search_info(d.handle, ctypes.byref(info), ctypes.byref(h), dims,
+                                   ctypes.byref(ni))
+            assert info.value == search
+            if search == 2:
+                print(f"grid cell {h.value:.2f} m, dims {list(dims)}, {ni.value} entries")
+                assert dims[2] > 1  # a 3-D grid, not columns
+    (gb, nb), (gg, ng) = out[1], out[2]
+    _, d = P.make_hit_buffers(capacity=1 << 20)
+    with d:
+        gpu_run(P, d, wl)
+        info = ctypes.c_int32()
+        gpu_lib.pp_search_info(d.handle, ctypes.byref(info), None, None, None)
+        assert info.value == 2
+    assert nb == ng == 100_000 and len(gb["wavelength"]) > 1000
+    for k in gb:
+        assert gb[k].tobytes() == gg[k].tobytes(), k
+    r, rsim = oracle_run(P, O, wl)
+    assert rsim == 100_000
+    n, d = compare_photon_by_photon(gg, r, module_diff_frac=0.01)
+    print(f"lattice: {n} common hits, {d} differing")
+
+
 def test_grid_search_1000_modules_matches_oracle(P, O, dev):
     """Uneven multi-source batch over 1000 modules (grid search by default)
     photon by photon against the oracle's brute-force scan."""
