@@ -225,13 +225,16 @@ __device__ __forceinline__ bool store_hits(bool has_hit, const HitRec& h, uint32
 // ---------------------------------------------------------------------------
 using pp::lds_bytes;
 
-enum { kSearchBrute = 0, kSearchGrid = 1, kSearchMixed = 2 };
+// kSearchGridGlobal: the grid search on tables read from global memory (L2
+// and the vector cache) instead of LDS copies, for target lists whose records
+// and grid tables exceed the LDS
+enum { kSearchBrute = 0, kSearchGrid = 1, kSearchMixed = 2, kSearchGridGlobal = 3 };
 
 // LDS tables of one workgroup: targets, grid cells and items, spectrum knots
 struct Lds {
-    float4* tgt;
-    float4* cells;
-    uint16_t* items;
+    const float4* tgt;
+    const float4* cells;
+    const uint16_t* items;
     float* spec;
 };
 
@@ -239,16 +242,30 @@ template <int SEARCH>
 __device__ __forceinline__ Lds stage_lds(const KParams& p) {
     extern __shared__ float4 lds[];
     Lds l;
+    if constexpr (SEARCH == kSearchGridGlobal) {
+        // only the spectrum is staged; the search reads the device tables
+        l.tgt = p.targets;
+        l.cells = p.cells;
+        l.items = p.cell_items;
+        l.spec = reinterpret_cast<float*>(lds);
+        for (int i = threadIdx.x; i < p.spec_n; i += blockDim.x) l.spec[i] = p.spec[i];
+        __syncthreads();
+        return l;
+    }
     // a mixed (sphere and planar) list has two records per target
     const int n_rec = SEARCH == kSearchMixed ? 2 * p.n_targets : p.n_targets;
-    l.tgt = lds;
-    l.cells = lds + n_rec;
-    l.items = reinterpret_cast<uint16_t*>(l.cells + (SEARCH == kSearchGrid ? 2 * p.grid.n_cells : 0));
-    l.spec = reinterpret_cast<float*>(l.items + (SEARCH == kSearchGrid ? ((p.grid.n_items + 1) & ~1) : 0));
-    for (int i = threadIdx.x; i < n_rec; i += blockDim.x) l.tgt[i] = p.targets[i];
+    float4* const tgt = lds;
+    float4* const cells = lds + n_rec;
+    uint16_t* const items =
+        reinterpret_cast<uint16_t*>(cells + (SEARCH == kSearchGrid ? 2 * p.grid.n_cells : 0));
+    l.tgt = tgt;
+    l.cells = cells;
+    l.items = items;
+    l.spec = reinterpret_cast<float*>(items + (SEARCH == kSearchGrid ? ((p.grid.n_items + 1) & ~1) : 0));
+    for (int i = threadIdx.x; i < n_rec; i += blockDim.x) tgt[i] = p.targets[i];
     if (SEARCH == kSearchGrid) {
-        for (int i = threadIdx.x; i < 2 * p.grid.n_cells; i += blockDim.x) l.cells[i] = p.cells[i];
-        for (int i = threadIdx.x; i < p.grid.n_items; i += blockDim.x) l.items[i] = p.cell_items[i];
+        for (int i = threadIdx.x; i < 2 * p.grid.n_cells; i += blockDim.x) cells[i] = p.cells[i];
+        for (int i = threadIdx.x; i < p.grid.n_items; i += blockDim.x) items[i] = p.cell_items[i];
     }
     for (int i = threadIdx.x; i < p.spec_n; i += blockDim.x) l.spec[i] = p.spec[i];
     __syncthreads();
@@ -257,12 +274,12 @@ __device__ __forceinline__ Lds stage_lds(const KParams& p) {
 
 template <int SEARCH>
 using SearchT = typename std::conditional<
-    SEARCH == kSearchGrid, pp::GridSearch,
+    SEARCH == kSearchGrid || SEARCH == kSearchGridGlobal, pp::GridSearch,
     typename std::conditional<SEARCH == kSearchMixed, pp::MixedSearch, pp::BruteSearch>::type>::type;
 
 template <int SEARCH>
 __device__ __forceinline__ SearchT<SEARCH> make_search(const KParams& p, const Lds& l) {
-    if constexpr (SEARCH == kSearchGrid) {
+    if constexpr (SEARCH == kSearchGrid || SEARCH == kSearchGridGlobal) {
         return pp::GridSearch{l.tgt, l.cells, l.items, p.grid};
     } else if constexpr (SEARCH == kSearchMixed) {
         return pp::MixedSearch{l.tgt, p.n_targets};
@@ -608,6 +625,7 @@ struct pp_ctx {
     // uniform grid over the targets, rebuilt when targets or cell size change
     pp::GridDesc grid{};
     float grid_cell_built = -1.0f;  // requested cell size of the built grid
+    bool grid_global = false;        // the built grid's tables exceed the LDS: read from memory
     float grid_cell_z_built = 0.0f;
     float4* d_cells = nullptr;
     uint16_t* d_cell_items = nullptr;
@@ -645,6 +663,7 @@ int set_dev(pp_ctx* c) {
 
 constexpr int kMaxLdsBytes = 160 * 1024;
 constexpr size_t kGridLdsBudget = 64 * 1024;  // grid tables + targets + spectrum per block
+constexpr size_t kGridGlobalBudget = (size_t)64 << 20;  // tables read from device memory
 
 // (Re)build the target grid for the requested cell edge and upload it.
 int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
@@ -656,12 +675,18 @@ int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
     }
     pp::GridTables t;
     // the occupancy-friendly budget first; for many targets (their records
-    // alone may fill it) the whole LDS, at fewer workgroups per CU
+    // alone may fill it) the whole LDS, at fewer workgroups per CU; beyond
+    // that, tables in device memory that the search reads through the caches
+    bool global = false;
     if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
                                c->spec_n, kGridLdsBudget, t) &&
         !pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
-                               c->spec_n, kMaxLdsBytes, t))
-        return fail(PP_ERR_INVALID, "grid: no cell size fits the LDS budget");
+                               c->spec_n, kMaxLdsBytes, t)) {
+        global = true;
+        if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req,
+                                   cell_z_req, c->spec_n, kGridGlobalBudget, t))
+            return fail(PP_ERR_INVALID, "grid: no cell size fits");
+    }
     if (t.cells.size() > c->cell_cap) {
         if (c->d_cells) PP_HIP(hipFree(c->d_cells));
         c->d_cells = nullptr;
@@ -683,6 +708,7 @@ int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
                           hipMemcpyHostToDevice, c->stream));
     PP_HIP(hipStreamSynchronize(c->stream));
     c->grid = t.g;
+    c->grid_global = global;
     c->grid_cell_built = cell_req;
     c->grid_cell_z_built = cell_z_req;
     return PP_OK;
@@ -727,9 +753,11 @@ int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
 #define PP_K(PH, S, C) propagate_kernel<PH, S, C>
 #define PP_K3(S, C) {PP_K(PP_PHASE_HG, S, C), PP_K(PP_PHASE_MIXED_HG_ES, S, C), PP_K(PP_PHASE_MIXED_HG_SL, S, C)}
     // [count][search kind][phase kind]
-    static const KernelFn kKernels[2][3][3] = {
-        {PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false), PP_K3(kSearchMixed, false)},
-        {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true), PP_K3(kSearchMixed, true)}};
+    static const KernelFn kKernels[2][4][3] = {
+        {PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false), PP_K3(kSearchMixed, false),
+         PP_K3(kSearchGridGlobal, false)},
+        {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true), PP_K3(kSearchMixed, true),
+         PP_K3(kSearchGridGlobal, true)}};
 #undef PP_K3
 #undef PP_K
     const KernelFn kern = kKernels[count][skind][phase];
@@ -760,9 +788,11 @@ int launch_batch(pp_ctx* c, const KParams& kp, const pp_source* sources,
     using BatchFn = void (*)(KParams, BatchParams);
 #define PP_K(PH, S, C) propagate_batch_kernel<PH, S, C>
 #define PP_K3(S, C) {PP_K(PP_PHASE_HG, S, C), PP_K(PP_PHASE_MIXED_HG_ES, S, C), PP_K(PP_PHASE_MIXED_HG_SL, S, C)}
-    static const BatchFn kKernels[2][3][3] = {
-        {PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false), PP_K3(kSearchMixed, false)},
-        {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true), PP_K3(kSearchMixed, true)}};
+    static const BatchFn kKernels[2][4][3] = {
+        {PP_K3(kSearchBrute, false), PP_K3(kSearchGrid, false), PP_K3(kSearchMixed, false),
+         PP_K3(kSearchGridGlobal, false)},
+        {PP_K3(kSearchBrute, true), PP_K3(kSearchGrid, true), PP_K3(kSearchMixed, true),
+         PP_K3(kSearchGridGlobal, true)}};
 #undef PP_K3
 #undef PP_K
     const BatchFn kern = kKernels[count][skind][phase];
@@ -1070,8 +1100,11 @@ int pp_set_targets(pp_ctx* ctx, const pp_target* t, int n) {
     // records per target: (x, y, z, r^2), plus (kind bits, lx/2, ly/2, 0) in a
     // list with planar targets
     const int per = planar ? 2 : 1;
-    if ((size_t)n * per * sizeof(float4) + 8192 * sizeof(float) > (size_t)kMaxLdsBytes)
-        return fail(PP_ERR_INVALID, "pp_set_targets: too many targets for LDS staging");
+    // planar lists are scanned from LDS; sphere lists of any length up to
+    // kMaxTargets are searched by the grid, from device memory if need be
+    if (planar && (size_t)n * per * sizeof(float4) + 8192 * sizeof(float) > (size_t)kMaxLdsBytes)
+        return fail(PP_ERR_INVALID, "pp_set_targets: too many planar targets for LDS staging");
+    if (n > kMaxTargets) return fail(PP_ERR_INVALID, "pp_set_targets: more than 65535 targets");
     if (int rc = set_dev(ctx)) return rc;
     std::vector<float4> h((size_t)std::max(n * per, 1));
     for (int i = 0; i < n; ++i) {
@@ -1166,10 +1199,14 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
         }
     }
     const bool grid = search == PP_SEARCH_GRID;
-    const int skind = grid ? kSearchGrid : ctx->planar ? kSearchMixed : kSearchBrute;
-    const size_t lds = lds_bytes(ctx->planar ? 2 * ctx->n_targets : ctx->n_targets,
-                                 grid ? ctx->grid.n_cells : 0,
-                                 grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
+    const bool global = grid && ctx->grid_global;
+    const int skind = global ? kSearchGridGlobal
+                      : grid ? kSearchGrid : ctx->planar ? kSearchMixed : kSearchBrute;
+    // the global grid stages only the spectrum
+    const size_t lds = global ? std::max<size_t>((size_t)ctx->spec_n * 4, 16)
+                              : lds_bytes(ctx->planar ? 2 * ctx->n_targets : ctx->n_targets,
+                                          grid ? ctx->grid.n_cells : 0,
+                                          grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
     if (lds > (size_t)kMaxLdsBytes)
         return fail(PP_ERR_INVALID, "pp_propagate: targets and tables exceed the LDS budget");
     const int phase = ctx->medium.phase.kind;

@@ -73,8 +73,8 @@ __host__ __device__ __forceinline__ float rcp_clamped(float x) {
 #endif
 }
 
-// a * b for 0 <= a, b and a * b < 2^24 (cell indices: a grid has fewer
-// cells than fit in LDS): the full-rate 24-bit multiply on the device
+// a * b for 0 <= a, b and a * b < 2^24 (cell indices: a grid has at most
+// 2^21 cells, its tables' 64 MB budget): the full-rate 24-bit multiply
 __host__ __device__ __forceinline__ int mul24(int a, int b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __umul24(a, b);

@@ -378,6 +378,41 @@ def test_lattice_of_4096_spheres(P, O, gpu_lib):
     print(f"lattice: {n} common hits, {d} differing")
 
 
+def test_16000_spheres_from_device_memory(P, O, gpu_lib):
+    """16000 spheres (a 20 x 20 x 40 lattice): records and grid tables exceed
+    the LDS, so the automatic search walks a grid read from device memory. It
+    agrees photon by photon with the oracle's every-target scan; the
+    brute-force scan on the device, which needs the records in LDS, is
+    refused with a message."""
+    import ctypes
+    import itertools
+    medium = P.CascadiaMediumProperties()
+    xs, zs = np.arange(20) * 5.0 - 47.5, np.arange(40) * 5.0 - 97.5
+    targets = [P.spherical_target((x, y, z), 0.6, i + 1)
+               for i, (x, y, z) in enumerate(itertools.product(xs, xs, zs))]
+    src = P.PointlikeIsotropicEmitter(np.float32([1.3, -0.4, 2.1]), 0.0, 30_000)
+    wl = P.workloads.Workload("lattice", [src], targets, medium, P.Monochromatic(420.0), 22)
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 1
+    _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
+    with d:
+        g, n = gpu_run(P, d, wl)
+        info, h, ni = ctypes.c_int32(), ctypes.c_float(), ctypes.c_int32()
+        dims = (ctypes.c_int32 * 3)()
+        gpu_lib.pp_search_info(d.handle, ctypes.byref(info), ctypes.byref(h), dims, ctypes.byref(ni))
+        print(f"grid cell {h.value:.2f} m, dims {list(dims)}, {ni.value} entries")
+        assert info.value == 2 and ni.value >= 16000
+    assert n == 30_000 and len(g["wavelength"]) > 1000
+    r, rsim = oracle_run(P, O, wl)
+    cn, dn = compare_photon_by_photon(g, r, module_diff_frac=0.01)
+    print(f"16000 spheres: {cn} common hits, {dn} differing")
+    cfg.search = 1
+    _, d2 = P.make_hit_buffers(capacity=1 << 20, config=cfg)
+    with d2:
+        with pytest.raises(P._lib.PhotonPropLibraryError, match="LDS"):
+            gpu_run(P, d2, wl)
+
+
 def test_grid_search_1000_modules_matches_oracle(P, O, dev):
     """Uneven multi-source batch over 1000 modules (grid search by default)
     photon by photon against the oracle's brute-force scan."""
