@@ -251,8 +251,9 @@ int pp_set_spectrum(pp_ctx* ctx, float mono_wl, const float* table, int n, int m
 /* Targets uploaded to the device (src/photon_prop_cuda.jl:911). Up to 65535
  * targets. Spheres are searched through a uniform grid held in LDS, or read
  * from device memory when the records and tables exceed it; lists with
- * planar targets are scanned from LDS (about 4900 targets at most), as is a
- * list under PP_SEARCH_BRUTE (about 9700 spheres). */
+ * planar targets are scanned from LDS (4096 targets at most, two 16-byte
+ * records each beside a spectrum of up to 8192 knots), as is a list under
+ * PP_SEARCH_BRUTE (about 8000 spheres, fewer with a long spectrum). */
 int pp_set_targets(pp_ctx* ctx, const pp_target* targets, int n);
 
 /* run_photon_prop_no_local_cache! (src/photon_prop_cuda.jl:889-959) without
