@@ -637,6 +637,9 @@ struct pp_ctx {
     int32_t n_launch = 0;
     bool perm_valid = false;
     std::vector<int64_t> perm;
+    // pinned host staging of hit records for pp_copy_hits
+    HitRecord* h_rec = nullptr;
+    int64_t h_rec_cap = 0;
     // source table of the persistent batch kernel: device copy and pinned
     // host staging, laid out sources | photon counts | chunk prefix
     void* d_batch = nullptr;
@@ -1010,6 +1013,7 @@ int pp_destroy(pp_ctx* ctx) {
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     if (ctx->d_batch) (void)hipFree(ctx->d_batch);
     if (ctx->h_batch) (void)hipHostFree(ctx->h_batch);
+    if (ctx->h_rec) (void)hipHostFree(ctx->h_rec);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1409,19 +1413,35 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
     const bool ordered = ctx->cfg.deterministic_order != 0;
     // records to read: [lo, lo+cnt) of the device buffer
     const int64_t lo = ordered ? 0 : first, cnt = ordered ? ctx->n_hits : n;
-    std::vector<HitRecord> rec((size_t)cnt);
+    // records through a pinned staging buffer (kept by the context)
+    if (cnt > ctx->h_rec_cap) {
+        if (ctx->h_rec) PP_HIP(hipHostFree(ctx->h_rec));
+        ctx->h_rec = nullptr;
+        ctx->h_rec_cap = 0;
+        PP_HIP(hipHostMalloc((void**)&ctx->h_rec, (size_t)cnt * sizeof(HitRecord), hipHostMallocDefault));
+        ctx->h_rec_cap = cnt;
+    }
+    const HitRecord* rec = ctx->h_rec;
+    PP_HIP(hipMemcpyAsync(ctx->h_rec, ctx->hits.rec + lo, (size_t)cnt * sizeof(HitRecord),
+                          hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
-    PP_HIP(hipMemcpy(rec.data(), ctx->hits.rec + lo, (size_t)cnt * sizeof(HitRecord),
-                     hipMemcpyDeviceToHost));
     const uint64_t pid_mask = kMaxPhotonsPerSource - 1;
     if (ordered && !ctx->perm_valid) {
-        ctx->perm.resize((size_t)ctx->n_hits);
-        std::iota(ctx->perm.begin(), ctx->perm.end(), (int64_t)0);
-        std::sort(ctx->perm.begin(), ctx->perm.end(), [&](int64_t a, int64_t b) {
-            const HitRecord &ra = rec[(size_t)a], &rb = rec[(size_t)b];
-            return ra.source_ix != rb.source_ix ? ra.source_ix < rb.source_ix
-                                                : (ra.id & pid_mask) < (rb.id & pid_mask);
+        // sort (source, photon id) keys beside their record index: one
+        // contiguous array instead of indirect two-field compares
+        struct Key {
+            int32_t src;
+            uint64_t pid;
+            int64_t idx;
+        };
+        std::vector<Key> keys((size_t)ctx->n_hits);
+        for (int64_t k = 0; k < ctx->n_hits; ++k)
+            keys[(size_t)k] = {rec[k].source_ix, rec[k].id & pid_mask, k};
+        std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+            return a.src != b.src ? a.src < b.src : a.pid < b.pid;
         });
+        ctx->perm.resize((size_t)ctx->n_hits);
+        for (int64_t k = 0; k < ctx->n_hits; ++k) ctx->perm[(size_t)k] = keys[(size_t)k].idx;
         ctx->perm_valid = true;
     }
     for (int64_t k = 0; k < n; ++k) {
