@@ -272,7 +272,10 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
  * to skip that column. pos3/dir3/init_dir3 are n x 3 row-major. time is float
  * if time_f64 == 0, else double. module_ix is the 1-based target index.
  * photon_id / source_ix identify the emitting photon (deterministic key).
- * With cfg->deterministic_order the records are ordered by (source, photon). */
+ * With cfg->deterministic_order the records are ordered by (source, photon):
+ * the first such copy after a propagation sorts them on the device (a radix
+ * sort into a buffer the context keeps until pp_destroy); later copies of the
+ * same propagation reuse it. */
 int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3,
                  float* init_dir3, float* wavelength, void* time, int time_f64,
                  float* dist_travelled, int64_t* module_ix, int32_t* n_steps,

@@ -24,6 +24,8 @@
 #include "pp_grid_build.h"
 #include "pp_search.h"
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 using pp::f3;
 
 // ---------------------------------------------------------------------------
@@ -640,6 +642,12 @@ struct pp_ctx {
     // pinned host staging of hit records for pp_copy_hits
     HitRecord* h_rec = nullptr;
     int64_t h_rec_cap = 0;
+    // device sort of the hits by (source, photon id): the records in that
+    // order, keys, indices and the radix sort's scratch, in one allocation
+    char* d_sort = nullptr;
+    size_t d_sort_cap = 0;
+    bool sorted_valid = false;  // d_sort holds the sorted records of the last propagation
+    int32_t last_n_sources = 0;
     // source table of the persistent batch kernel: device copy and pinned
     // host staging, laid out sources | photon counts | chunk prefix
     void* d_batch = nullptr;
@@ -1014,6 +1022,7 @@ int pp_destroy(pp_ctx* ctx) {
     if (ctx->d_batch) (void)hipFree(ctx->d_batch);
     if (ctx->h_batch) (void)hipHostFree(ctx->h_batch);
     if (ctx->h_rec) (void)hipHostFree(ctx->h_rec);
+    if (ctx->d_sort) (void)hipFree(ctx->d_sort);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1263,6 +1272,8 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     ctx->work_tests = (int64_t)hc.tests;
     ctx->work_cells = (int64_t)hc.cells;
     ctx->perm_valid = false;
+    ctx->sorted_valid = false;
+    ctx->last_n_sources = n_sources;
     if (n_hits_out) *n_hits_out = ctx->n_hits;
     if (n_ph_sim_out) *n_ph_sim_out = ctx->n_ph_sim;
     if (hc.err) return fail(PP_ERR_HIT_OVERFLOW, "hit buffer overflow (PhotonPropOOMException)");
@@ -1363,6 +1374,7 @@ int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t s
     ctx->n_hits = 0;  // the tuning runs' hits are not results
     ctx->n_ph_sim = 0;
     ctx->perm_valid = false;
+    ctx->sorted_valid = false;
     if (best) *best = cur;
     if (best_ms) *best_ms = t_best;
     return PP_OK;
@@ -1401,6 +1413,83 @@ int pp_search_info(pp_ctx* ctx, int32_t* search, float* cell, int32_t* dims, int
     return PP_OK;
 }
 
+// Sort key of hit k: the source index above the 40 photon-id bits, so that an
+// ascending sort orders by source, then by photon id; the value is k.
+__global__ void hit_keys_kernel(const HitRecord* __restrict__ rec, int64_t n,
+                                uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    key[k] = ((uint64_t)(uint32_t)rec[k].source_ix << kPidBits) |
+             (rec[k].id & (kMaxPhotonsPerSource - 1));
+    idx[k] = (uint32_t)k;
+}
+
+// out[k] = rec[idx[k]], 64 bytes per lane as four 16-byte loads and stores.
+__global__ void gather_hits_kernel(const HitRecord* __restrict__ rec,
+                                   const uint32_t* __restrict__ idx, int64_t n,
+                                   HitRecord* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint4* src = (const uint4*)(rec + idx[k]);
+    uint4* dst = (uint4*)(out + k);
+    const uint4 a = src[0], b = src[1], c = src[2], d = src[3];
+    dst[0] = a;
+    dst[1] = b;
+    dst[2] = c;
+    dst[3] = d;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The hits in (source, photon id) order, written by the device to the front
+// of ctx->d_sort: a radix sort over 40 + bits(n_sources - 1) key bits carries
+// the record indices (the sort is stable; a photon hits at most once, so keys
+// are unique), then a gather writes the records in that order.  Sets
+// *done = false and does nothing when the keys do not fit 64 bits, the
+// indices 32 bits or the buffer device memory; the caller then sorts on the
+// host.
+static int sort_hits_device(pp_ctx* ctx, bool* done) {
+    *done = false;
+    const int64_t n = ctx->n_hits;
+    const int32_t ns = ctx->last_n_sources;
+    if (n <= 0 || n > INT32_MAX || ns <= 0 || ns > (1 << 24)) return PP_OK;
+    int end_bit = (int)kPidBits;
+    for (uint32_t v = (uint32_t)(ns - 1); v; v >>= 1) ++end_bit;
+    size_t temp = 0;
+    PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint64_t*)nullptr,
+                                              (uint64_t*)nullptr, (const uint32_t*)nullptr,
+                                              (uint32_t*)nullptr, (int)n, 0, end_bit, ctx->stream));
+    const size_t br = align256((size_t)n * sizeof(HitRecord));
+    const size_t bk = align256((size_t)n * 8), bv = align256((size_t)n * 4);
+    const size_t need = br + 2 * bk + 2 * bv + align256(temp);
+    if (need > ctx->d_sort_cap) {
+        if (ctx->d_sort) PP_HIP(hipFree(ctx->d_sort));
+        ctx->d_sort = nullptr;
+        ctx->d_sort_cap = 0;
+        if (hipMalloc((void**)&ctx->d_sort, need) != hipSuccess) {
+            (void)hipGetLastError();  // out of device memory: sort on the host
+            ctx->d_sort = nullptr;
+            return PP_OK;
+        }
+        ctx->d_sort_cap = need;
+    }
+    char* base = ctx->d_sort + br;
+    uint64_t* k_in = (uint64_t*)base;
+    uint64_t* k_out = (uint64_t*)(base + bk);
+    uint32_t* v_in = (uint32_t*)(base + 2 * bk);
+    uint32_t* v_out = (uint32_t*)(base + 2 * bk + bv);
+    void* tmp = base + 2 * bk + 2 * bv;
+    hit_keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(ctx->hits.rec, n, k_in, v_in);
+    PP_HIP(hipGetLastError());
+    PP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, v_out, (int)n, 0,
+                                              end_bit, ctx->stream));
+    gather_hits_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+        ctx->hits.rec, v_out, n, (HitRecord*)ctx->d_sort);
+    PP_HIP(hipGetLastError());
+    *done = true;
+    return PP_OK;
+}
+
 int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3,
                  float* init_dir3, float* wavelength, void* time, int time_f64,
                  float* dist_travelled, int64_t* module_ix, int32_t* n_steps,
@@ -1411,8 +1500,16 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
     if (n == 0) return PP_OK;
     if (int rc = set_dev(ctx)) return rc;
     const bool ordered = ctx->cfg.deterministic_order != 0;
-    // records to read: [lo, lo+cnt) of the device buffer
-    const int64_t lo = ordered ? 0 : first, cnt = ordered ? ctx->n_hits : n;
+    if (ordered && !ctx->sorted_valid && !ctx->perm_valid) {
+        bool done = false;
+        if (int rc = sort_hits_device(ctx, &done)) return rc;
+        ctx->sorted_valid = done;
+    }
+    // the host sorts only where the device could not: it then reads every record
+    const bool host_sort = ordered && !ctx->sorted_valid;
+    // records to read: [lo, lo+cnt) of the device buffer src
+    const HitRecord* src = ordered && ctx->sorted_valid ? (const HitRecord*)ctx->d_sort : ctx->hits.rec;
+    const int64_t lo = host_sort ? 0 : first, cnt = host_sort ? ctx->n_hits : n;
     // records through a pinned staging buffer (kept by the context)
     if (cnt > ctx->h_rec_cap) {
         if (ctx->h_rec) PP_HIP(hipHostFree(ctx->h_rec));
@@ -1422,11 +1519,11 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
         ctx->h_rec_cap = cnt;
     }
     const HitRecord* rec = ctx->h_rec;
-    PP_HIP(hipMemcpyAsync(ctx->h_rec, ctx->hits.rec + lo, (size_t)cnt * sizeof(HitRecord),
+    PP_HIP(hipMemcpyAsync(ctx->h_rec, src + lo, (size_t)cnt * sizeof(HitRecord),
                           hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
     const uint64_t pid_mask = kMaxPhotonsPerSource - 1;
-    if (ordered && !ctx->perm_valid) {
+    if (host_sort && !ctx->perm_valid) {
         // sort (source, photon id) keys beside their record index: one
         // contiguous array instead of indirect two-field compares
         struct Key {
@@ -1445,7 +1542,7 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
         ctx->perm_valid = true;
     }
     for (int64_t k = 0; k < n; ++k) {
-        const HitRecord& r = rec[(size_t)(ordered ? ctx->perm[(size_t)(first + k)] : k)];
+        const HitRecord& r = rec[(size_t)(host_sort ? ctx->perm[(size_t)(first + k)] : k)];
         for (int c = 0; c < 3; ++c) {
             if (pos3) pos3[3 * k + c] = r.f[c];
             if (dir3) dir3[3 * k + c] = r.f[3 + c];

@@ -158,6 +158,45 @@ def test_bit_exact_repeat_and_launch_shape_independence(P, dev):
         assert g1[k].tobytes() == g3[k].tobytes(), k
 
 
+def test_ordered_copy_in_chunks(P, gpu_lib):
+    """Ordered copies of sub-ranges, with an unordered copy in between, give
+    the rows of one ordered copy of every hit; the unordered copy gives the
+    same multiset of hits."""
+    import ctypes
+    wl = P.workloads.uneven_batch_1000(n_sources=6, seed=5, min_photons=1e5, max_photons=2e5)
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 1
+    _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
+    with d:
+        g, _ = gpu_run(P, d, wl)
+        n = len(g["photon_id"])
+        assert n > 100
+
+        def copy(first, cnt):
+            pid = np.zeros(cnt, np.uint64)
+            src = np.zeros(cnt, np.int32)
+            pos = np.zeros((cnt, 3), np.float32)
+            assert gpu_lib.pp_copy_hits(d.handle, first, cnt, P._lib.fptr(pos), None, None, None,
+                                        None, 0, None, None, None, P._lib.fptr(pid),
+                                        P._lib.fptr(src)) == 0
+            return pid, src, pos
+
+        a = copy(0, n // 3)
+        cfg.deterministic_order = 0
+        assert gpu_lib.pp_set_config(d.handle, ctypes.byref(cfg)) == 0
+        u = copy(0, n)
+        cfg.deterministic_order = 1
+        assert gpu_lib.pp_set_config(d.handle, ctypes.byref(cfg)) == 0
+        b = copy(n // 3, n - n // 3)
+        for k, part in enumerate((a, b)):
+            lo = 0 if k == 0 else n // 3
+            sl = slice(lo, lo + len(part[0]))
+            assert part[0].tobytes() == g["photon_id"][sl].tobytes()
+            assert part[1].tobytes() == g["source_ix"][sl].astype(np.int32).tobytes()
+        keys = lambda p, s: sorted(zip(s.tolist(), p.tolist()))
+        assert keys(u[0], u[1]) == keys(np.concatenate([a[0], b[0]]), np.concatenate([a[1], b[1]]))
+
+
 def test_hit_buffer_overflow_raises(P, O, gpu_lib):
     wl = P.workloads.readme_point_source(photons=1_000_000)
     _, small = P.make_hit_buffers(capacity=8)
