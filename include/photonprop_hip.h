@@ -165,7 +165,8 @@ typedef struct pp_config {
     float grid_cell;            /* grid cell edge (m); 0 = auto: the edge (and,
                                    with grid_cell_z = 0, the shape) of least
                                    modelled walk cost                          */
-    int32_t deterministic_order;/* pp_copy_hits sorts by (source, photon id)   */
+    int32_t deterministic_order;/* pp_copy_hits sorts by (source, photon id):
+                                   0 = no, 1 = on the device, 2 = on the host */
     int32_t count_work;         /* instrumented kernel: counts steps, sphere
                                    tests and grid-cell visits (pp_work_counts) */
     int32_t batch_mode;         /* enum pp_batch_mode                          */
@@ -273,9 +274,11 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
  * if time_f64 == 0, else double. module_ix is the 1-based target index.
  * photon_id / source_ix identify the emitting photon (deterministic key).
  * With cfg->deterministic_order the records are ordered by (source, photon):
- * the first such copy after a propagation sorts them on the device (a radix
- * sort into a buffer the context keeps until pp_destroy); later copies of the
- * same propagation reuse it. */
+ * with 1 the first such copy after a propagation sorts them on the device (a
+ * radix sort into a buffer of about 88 bytes per hit that the context keeps
+ * until pp_destroy; on the host beyond 2^24 sources or 2^31 hits or when that
+ * buffer does not fit), with 2 on the host (every record is copied each
+ * call); later copies of the same propagation reuse the order. */
 int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3,
                  float* init_dir3, float* wavelength, void* time, int time_f64,
                  float* dist_travelled, int64_t* module_ix, int32_t* n_steps,

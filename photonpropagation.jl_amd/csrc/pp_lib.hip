@@ -743,6 +743,8 @@ int normalise_config(pp_config& c, const char* who) {
         return fail(PP_ERR_INVALID, w + ": unknown batch_mode");
     if (c.chunk_photons < 0 || c.chunk_photons % 64 || c.chunk_photons > (1 << 24))
         return fail(PP_ERR_INVALID, w + ": chunk_photons must be a multiple of 64 in [0, 2^24]");
+    if (c.deterministic_order < 0 || c.deterministic_order > 2)
+        return fail(PP_ERR_INVALID, w + ": deterministic_order must be 0, 1 or 2");
     return PP_OK;
 }
 
@@ -1499,16 +1501,18 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
         return fail(PP_ERR_INVALID, "pp_copy_hits: range outside [0, n_hits)");
     if (n == 0) return PP_OK;
     if (int rc = set_dev(ctx)) return rc;
+    // deterministic_order: 0 = buffer order, 1 = sorted on the device (host
+    // where the device cannot), 2 = sorted on the host (no device buffer)
     const bool ordered = ctx->cfg.deterministic_order != 0;
-    if (ordered && !ctx->sorted_valid && !ctx->perm_valid) {
+    if (ordered && ctx->cfg.deterministic_order != 2 && !ctx->sorted_valid && !ctx->perm_valid) {
         bool done = false;
         if (int rc = sort_hits_device(ctx, &done)) return rc;
         ctx->sorted_valid = done;
     }
     // the host sorts only where the device could not: it then reads every record
-    const bool host_sort = ordered && !ctx->sorted_valid;
+    const bool host_sort = ordered && (ctx->cfg.deterministic_order == 2 || !ctx->sorted_valid);
     // records to read: [lo, lo+cnt) of the device buffer src
-    const HitRecord* src = ordered && ctx->sorted_valid ? (const HitRecord*)ctx->d_sort : ctx->hits.rec;
+    const HitRecord* src = ordered && !host_sort ? (const HitRecord*)ctx->d_sort : ctx->hits.rec;
     const int64_t lo = host_sort ? 0 : first, cnt = host_sort ? ctx->n_hits : n;
     // records through a pinned staging buffer (kept by the context)
     if (cnt > ctx->h_rec_cap) {

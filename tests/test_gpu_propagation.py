@@ -161,7 +161,7 @@ def test_bit_exact_repeat_and_launch_shape_independence(P, dev):
 def test_ordered_copy_in_chunks(P, gpu_lib):
     """Ordered copies of sub-ranges, with an unordered copy in between, give
     the rows of one ordered copy of every hit; the unordered copy gives the
-    same multiset of hits."""
+    same multiset of hits, and the host sort the device sort's rows."""
     import ctypes
     wl = P.workloads.uneven_batch_1000(n_sources=6, seed=5, min_photons=1e5, max_photons=2e5)
     cfg = P._lib.pp_config()
@@ -195,6 +195,15 @@ def test_ordered_copy_in_chunks(P, gpu_lib):
             assert part[1].tobytes() == g["source_ix"][sl].astype(np.int32).tobytes()
         keys = lambda p, s: sorted(zip(s.tolist(), p.tolist()))
         assert keys(u[0], u[1]) == keys(np.concatenate([a[0], b[0]]), np.concatenate([a[1], b[1]]))
+        # the host sort (deterministic_order = 2) gives the device sort's rows
+        cfg.deterministic_order = 2
+        assert gpu_lib.pp_set_config(d.handle, ctypes.byref(cfg)) == 0
+        h = copy(5, n - 7)
+        assert h[0].tobytes() == g["photon_id"][5:n - 2].tobytes()
+        assert h[1].tobytes() == g["source_ix"][5:n - 2].astype(np.int32).tobytes()
+        assert h[2].tobytes() == g["position"][5:n - 2].astype(np.float32).tobytes()
+        cfg.deterministic_order = 3
+        assert gpu_lib.pp_set_config(d.handle, ctypes.byref(cfg)) == P._lib.PP_ERR_INVALID
 
 
 def test_hit_buffer_overflow_raises(P, O, gpu_lib):
