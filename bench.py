@@ -83,6 +83,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=float, default=0.0,
                     help="photons for the CPU baseline (0 = calibrate to ~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", dest="legs", action="store_false",
+                    help="skip the host-copy, README-example and batching legs")
     ap.add_argument("--surrogate-kernels", action="store_true",
                     help="only time the surrogate's kernels without hipGraph (for kernel "
                          "tracing): 3 batched evaluations of 1e4 cascades x 1000 modules and "
@@ -115,29 +117,53 @@ def total_steps(hits, n_ph_sim, n_steps):
     return int(n_ph_sim) * n_steps - int(np.sum(n_steps - hits["n_steps"].astype(np.int64)))
 
 
-def cpu_baseline(wl_factory, photons):
+def cpu_model():
+    """Host CPU model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_run(wl, threads):
+    """One run of the Float32 port of the reference kernel on workload wl:
+    (seconds, photons, photon steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import photonprop as P
+    spec = wl.spectrum
+    kw = dict(n_steps=wl.n_steps, n_threads=threads, capacity=1 << 22)
+    if isinstance(spec, P.Monochromatic):
+        kw["mono_wl"] = spec.wavelength
+    else:
+        kw.update(table=spec.table, mapping=spec.mapping)
+    t = time.perf_counter()
+    _, nsim, steps = O.propagate_f32(P.processing.sources_to_pod(wl.sources), wl.medium.to_pod(),
+                                     P.targets_to_pod(wl.targets), wl.seed, **kw)
+    return time.perf_counter() - t, nsim, steps
+
+
+def cpu_baseline(wl_factory, photons, what):
+    """The Float32 port of the reference kernel (oracle/pp_ref_f32.c: Float32
+    arithmetic, libm float functions, every-target scan, 15 steps; OpenMP over
+    photons) on a bounded sample of the same workload, on all host threads:
+    median of three runs after one warm-up."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-
-    def once(n):
-        wl = wl_factory(n)
-        t = time.perf_counter()
-        O.propagate(P.processing.sources_to_pod(wl.sources), wl.medium.to_pod(),
-                    P.targets_to_pod(wl.targets), wl.seed, n_steps=wl.n_steps,
-                    table=wl.spectrum.table, mapping=wl.spectrum.mapping, n_threads=threads)
-        return time.perf_counter() - t
-
     if photons <= 0:
-        dt = once(200_000)
-        photons = max(200_000, int(200_000 * 10.0 / max(dt, 1e-3)))
-        photons = min(photons, 20_000_000)
-    once(min(photons, 200_000))  # warm-up
-    runs = sorted(photons / once(photons) for _ in range(3))
-    return dict(value=runs[1], unit="photons/s", cores=threads, kind="port",
-                sample=f"median of 3 oracle runs of {photons:.3g} photons of the same event "
-                       f"(brute-force 320-sphere search, 15 steps, OpenMP {threads} threads)")
+        dt, _, _ = cpu_run(wl_factory(200_000), threads)
+        photons = max(200_000, int(200_000 * 6.0 / max(dt, 1e-3)))
+        photons = min(photons, 50_000_000)
+    cpu_run(wl_factory(min(photons, 200_000)), threads)  # warm-up
+    runs = sorted(cpu_run(wl_factory(photons), threads) for _ in range(3))
+    dt, nsim, steps = runs[1]
+    return dict(value=nsim / dt, unit="photons/s", cores=threads, kind="port",
+                steps_per_s=steps / dt, model=cpu_model(),
+                sample=f"median of 3 runs of {photons:.3g} photons of {what} with the Float32 "
+                       f"port of the reference kernel (oracle/pp_ref_f32.c: every-target "
+                       f"scan, 15 steps, OpenMP {threads} threads)")
 
 
 def surrogate_kernels():
@@ -198,6 +224,140 @@ def pmt_leg():
                     "Philox block; vector-ALU bound, not HBM bound"}
 
 
+def copy_leg(lib, P, dev, pods, wl):
+    """Host-inclusive rate: one pp_propagate plus pp_copy_hits of every column
+    (the reference's call returns host columns, src/photon_prop_cuda.jl:
+    951-953), for each deterministic_order: 0 buffer order, 1 ordered on the
+    device, 2 ordered on the host. Wall clock of propagate + copy, median of
+    three after one warm-up (which allocates the pinned staging)."""
+    import ctypes
+
+    import numpy as np
+    f = P._lib.fptr
+    cfg = P._lib.pp_config()
+    P._lib.check(lib.pp_get_config(dev.handle, ctypes.byref(cfg)), "pp_get_config")
+    saved = cfg.deterministic_order
+    n_hits, n_sim = ctypes.c_int64(), ctypes.c_int64()
+    out = {}
+    for order in (0, 1, 2):
+        cfg.deterministic_order = order
+        P._lib.check(lib.pp_set_config(dev.handle, ctypes.byref(cfg)), "pp_set_config")
+        walls, copies = [], []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            P._lib.check(lib.pp_propagate(dev.handle, pods, len(pods), ctypes.c_uint64(wl.seed),
+                                          wl.n_steps, 1.0, ctypes.byref(n_hits),
+                                          ctypes.byref(n_sim)), "pp_propagate")
+            n = n_hits.value
+            cols = [np.empty((n, 3), np.float32), np.empty((n, 3), np.float32),
+                    np.empty((n, 3), np.float32), np.empty(n, np.float32),
+                    np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.int64),
+                    np.empty(n, np.int32), np.empty(n, np.uint64), np.empty(n, np.int32)]
+            t1 = time.perf_counter()
+            P._lib.check(lib.pp_copy_hits(dev.handle, 0, n, f(cols[0]), f(cols[1]), f(cols[2]),
+                                          f(cols[3]), f(cols[4]), 0, f(cols[5]), f(cols[6]),
+                                          f(cols[7]), f(cols[8]), f(cols[9])), "pp_copy_hits")
+            t2 = time.perf_counter()
+            walls.append(t2 - t0)
+            copies.append(t2 - t1)
+        w = sorted(walls[1:])[1]
+        out[f"order_{order}"] = {"propagate_plus_copy_ms": w * 1e3,
+                                 "copy_ms": sorted(copies[1:])[1] * 1e3, "hits": n,
+                                 "photons_per_s": n_sim.value / w}
+    cfg.deterministic_order = saved
+    P._lib.check(lib.pp_set_config(dev.handle, ctypes.byref(cfg)), "pp_set_config")
+    out["note"] = ("pp_propagate + pp_copy_hits of all 10 columns (position, direction, "
+                   "initial direction, wavelength, time, distance, module, steps, photon id, "
+                   "source) into host arrays; order 1 sorts on the device, 2 on the host")
+    return out
+
+
+def timed_propagations(lib, P, dev, pods, wl, warmup, steps):
+    """(wall ms per propagation, kernel ms, launches, n_ph_sim)."""
+    import ctypes
+
+    import numpy as np
+    n_hits, n_sim = ctypes.c_int64(), ctypes.c_int64()
+
+    def step():
+        P._lib.check(lib.pp_propagate(dev.handle, pods, len(pods), ctypes.c_uint64(wl.seed),
+                                      wl.n_steps, 1.0, ctypes.byref(n_hits),
+                                      ctypes.byref(n_sim)), "pp_propagate")
+    for _ in range(warmup):
+        step()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        kms.append(dev.last_timing()[0])
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, float(np.mean(kms)), int(dev.last_timing()[1]), n_sim.value
+
+
+def readme_leg(lib, P, with_cpu):
+    """The reference README's example (one isotropic point source at the
+    origin, 450 nm, one r = 0.3 m sphere at 10 m, Cascadia water): GPU at 1e5
+    photons (the example's size, launch-bound) and 1e8 photons, and the Float32
+    port of the reference kernel on the host at 1e5 photons."""
+    import numpy as np
+    out = {}
+    for photons, reps in ((100_000, 50), (100_000_000, 5)):
+        wl = P.workloads.readme_point_source(photons=photons)
+        cfg = P._lib.pp_config()
+        cfg.deterministic_order = 0
+        _, dev = P.make_hit_buffers(capacity=1 << 22, config=cfg)
+        with dev:
+            P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
+            pods = P.processing.sources_to_pod(wl.sources)
+            wall, kms, _, nsim = timed_propagations(lib, P, dev, pods, wl, 2, reps)
+            hits, _ = P.run_photon_prop_no_local_cache(wl.sources, wl.targets, wl.medium,
+                                                       wl.spectrum, wl.seed,
+                                                       P.HostHitBuffer(np.float32), dev)
+            st = total_steps(hits, nsim, wl.n_steps)
+        out[f"gpu_{photons:.0e}"] = {"photons_per_s": nsim / (wall * 1e-3),
+                                     "steps_per_s": st / (wall * 1e-3), "ms": wall,
+                                     "kernel_ms": kms, "hits": int(len(hits["n_steps"]))}
+    if with_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        wl = P.workloads.readme_point_source(photons=100_000)
+        cpu_run(wl, threads)
+        runs = sorted(cpu_run(wl, threads) for _ in range(3))
+        dt, nsim, st = runs[1]
+        out["cpu_1e+05"] = {"photons_per_s": nsim / dt, "steps_per_s": st / dt, "ms": dt * 1e3,
+                            "cores": threads, "kind": "port"}
+    return out
+
+
+def batching_leg(lib, P):
+    """The 256-source uneven batch (1000 modules): one launch per source (the
+    reference's schedule), one persistent work-stealing launch, and the same
+    as a hipGraph; then the autotuner's choice for the persistent launch."""
+    wl = P.workloads.uneven_batch_1000()
+    pods = P.processing.sources_to_pod(wl.sources)
+    out = {"photons": int(sum(s.photons for s in wl.sources)), "sources": len(pods)}
+    for mode, name in ((1, "per_source"), (2, "persistent"), (3, "graph")):
+        cfg = P._lib.pp_config()
+        cfg.batch_mode = mode
+        cfg.deterministic_order = 0
+        _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
+        with dev:
+            P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
+            wall, kms, launches, nsim = timed_propagations(lib, P, dev, pods, wl, 1, 5)
+        out[name] = {"ms": wall, "kernel_ms": kms, "launches": launches,
+                     "photons_per_s": nsim / (wall * 1e-3)}
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 0
+    _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
+    with dev:
+        P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
+        tuned = P.autotune.autotune(dev, wl.sources, wl.targets, wl.medium, wl.spectrum, wl.seed,
+                                    n_steps=wl.n_steps, sample_photons=4e7, use_cache=False)
+        wall, kms, _, nsim = timed_propagations(lib, P, dev, pods, wl, 1, 5)
+    out["tuned"] = {"ms": wall, "kernel_ms": kms, "photons_per_s": nsim / (wall * 1e-3),
+                    "config": tuned.get("config") if isinstance(tuned, dict) else None}
+    return out
+
+
 def pmt_kernels():
     print(json.dumps({"pmt_detection": pmt_leg()}), flush=True)
 
@@ -256,6 +416,31 @@ def surrogate_leg(args):
                   "steps_per_s": info["steps_per_s"],
                   "train_flop_per_step": 3.0 * flop_per_pair * info["batch"]},
     }
+
+
+def measured_counters(workload):
+    """VALU utilisation figures of the propagation kernel from the committed
+    rocprofv3 counter summary of this workload (lines `key: value` after the
+    'Read by bench.py' marker), or None."""
+    path = os.path.join(ROOT, "profiles", f"propagate_batch_kernel_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    out, on = {}, False
+    for line in open(path):
+        t = line.strip()
+        if t.startswith("Read by bench.py"):
+            on = True
+            continue
+        if on and ":" in t and not t.startswith("traffic_bytes_per_launch"):
+            k, v = t.split(":", 1)
+            try:
+                out[k.strip()] = float(v)
+            except ValueError:
+                pass
+    if not out:
+        return None
+    out["source"] = os.path.relpath(path, ROOT)
+    return out
 
 
 def measured_traffic(workload):
@@ -360,12 +545,16 @@ def main():
     steps, tests, cells = w_steps.value, w_tests.value, w_cells.value
     assert steps == total_steps(hits, nsim, wl.n_steps)
     n_t = len(wl.targets)
-    flop = (steps * (W_PHYS_FLOP + (W_GRID_STEP_FLOP if grid else 0.0)) + tests * W_ISEC_FLOP
-            + cells * W_CELL_FLOP)
-    transc = (steps * (W_PHYS_TRANSC + (W_GRID_STEP_TRANSC if grid else 0.0))
-              + tests * W_ISEC_TRANSC)
+    # algorithmic work: the reference step plus the sphere tests executed
+    flop = steps * W_PHYS_FLOP + tests * W_ISEC_FLOP
+    transc = steps * W_PHYS_TRANSC + tests * W_ISEC_TRANSC
+    # the grid's own clip and walk, reported apart (work of this build's
+    # search, not of the reference algorithm)
+    flop_grid = (steps * W_GRID_STEP_FLOP + cells * W_CELL_FLOP) if grid else 0.0
+    transc_grid = steps * W_GRID_STEP_TRANSC if grid else 0.0
     achieved = flop / (kms * 1e-3) / 1e12
     traffic, traffic_src = measured_traffic(wl.name)
+    counters = measured_counters(wl.name)
     ref_equiv = steps * (W_PHYS_FLOP + W_ISEC_FLOP * n_t) / (kms * 1e-3) / 1e12
     out = {
         "metric": "photons propagated/s on 1 MI355X; surrogate vs MC speedup at \u22645% "
@@ -405,12 +594,18 @@ def main():
                      "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
                      "transcendental_per_s": transc / (kms * 1e-3),
                      "flop_per_launch": flop,
+                     "mean_sphere_tests_per_step": tests / steps,
                      "reference_equivalent_tflops": ref_equiv,
-                     "basis": "FP32 ops executed: 47/step (reference step) + 25/step grid "
-                              "clip + 20/sphere test + 4/cell visit, counted by the "
-                              "instrumented kernel"},
+                     "grid_overhead": {"flop_per_launch": flop_grid,
+                                       "tflops": flop_grid / (kms * 1e-3) / 1e12,
+                                       "transcendental_per_s": transc_grid / (kms * 1e-3),
+                                       "basis": "25 FP32 ops + 3 reciprocals per step (clip), "
+                                                "4 per cell visit (walk)"},
+                     "basis": "photon steps x (47 + 20 T) FP32 ops, T = sphere tests executed "
+                              "per step, counted by the instrumented kernel (identical hits); "
+                              "transcendentals 7.4 + T per step",
+                     "counters": counters},
     }
-    dev.close()
     # issue rates measured on this device (8 chains per lane, 8 waves per SIMD)
     rates = {}
     for key, which, scale in (("fma_f32_tflops", P._lib.MB_FMA_F32, 1e12),
@@ -421,11 +616,16 @@ def main():
         rates[key] = r.value / scale
     out["roofline"]["measured_issue_rates"] = rates
     out["roofline"]["frac_of_measured_fma"] = achieved / rates["fma_f32_tflops"]
+    out["host_inclusive"] = copy_leg(lib, P, dev, pods, wl) if args.legs else None
+    dev.close()
+    out["readme_point_source"] = readme_leg(lib, P, not args.no_cpu) if args.legs else None
+    out["batching_uneven_1000"] = batching_leg(lib, P) if args.legs else None
     out["surrogate"] = surrogate_leg(args) if args.surrogate else None
     out["pmt_detection"] = pmt_leg() if args.pmt else None
     if not args.no_cpu and args.workload == "cascade":
         out["cpu_baseline"] = cpu_baseline(
-            lambda n: P.workloads.cascade_16_strings(photons=n), args.cpu_sample)
+            lambda n: P.workloads.cascade_16_strings(photons=n), args.cpu_sample,
+            "the same cascade event (320 spheres)")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

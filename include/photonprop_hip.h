@@ -387,6 +387,9 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *   PP_FN_PLANE_ISEC      in (kind, cx,cy,cz, a, b, px,py,pz, dx,dy,dz, step) out (hit,d)
  *                         kind as u32 bits (RECTANGLE: a, b = length_x, length_y;
  *                         DISC: a = radius)                      jl:353-404
+ *   PP_FN_MIXED_HGES      in (g, b, hg_fraction, u_choice, u) out (cos)  the
+ *                         kernel's branch-free MixedHGES sampler (HG with
+ *                         probability hg_fraction, else Einstein-Smoluchowski)
  * PP_FN_CASCADIA evaluates the context's medium (any kind): scattering
  * length, group velocity and phase index at wl. */
 enum pp_fn_id {
@@ -404,7 +407,8 @@ enum pp_fn_id {
     PP_FN_SPECTRUM = 11,
     PP_FN_SL_COS = 12,
     PP_FN_PLANE_ISEC = 13,
-    PP_FN_COUNT = 14
+    PP_FN_MIXED_HGES = 14,
+    PP_FN_COUNT = 15
 };
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out,
                int out_stride, int64_t n);

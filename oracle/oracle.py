@@ -19,10 +19,10 @@ LIB_PATH = os.path.join(HERE, "libpp_oracle.so")
 
 FN = dict(ISOTROPIC_DIR=0, ROT_FROM_EZ=1, HG_COS=2, ES_COS=3, UPDATE_DIR=4, SPHERE_ISEC=5,
           STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11, SL_COS=12,
-          PLANE_ISEC=13)
+          PLANE_ISEC=13, MIXED_HGES=14)
 FN_ARITY = {0: (2, 3), 1: (6, 3), 2: (2, 1), 3: (2, 1), 4: (5, 3), 5: (11, 2), 6: (2, 1),
             7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1), 12: (2, 1),
-            13: (13, 2)}
+            13: (13, 2), 14: (5, 1)}
 
 HIT_DTYPE = np.dtype([("pos", np.float32, 3), ("dir", np.float32, 3),
                       ("init_dir", np.float32, 3), ("wavelength", np.float32),
@@ -56,6 +56,10 @@ def lib():
         L.ppo_propagate.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
                                     c_uint64, c_int, c_float, c_void_p, c_int64,
                                     POINTER(c_int64), c_int]
+        L.ppf_propagate.restype = c_int64
+        L.ppf_propagate.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                    ctypes.c_uint64, c_int, c_float, c_void_p, c_int64,
+                                    c_void_p, c_void_p, c_int]
         L.ppo_photon_count.restype = c_int64
         L.ppo_photon_count.argtypes = [c_int64, c_float]
         L.ppo_quan_fry_n.restype = ctypes.c_double
@@ -155,6 +159,29 @@ def propagate(sources, medium, targets, seed, n_steps=15, photon_scaling=1.0, mo
     if n < 0:
         raise OverflowError("oracle hit capacity exceeded")
     return hits[:n].copy(), nsim.value
+
+
+def propagate_f32(sources, medium, targets, seed, n_steps=15, photon_scaling=1.0, mono_wl=0.0,
+                  table=None, mapping=0, capacity=1 << 20, n_threads=0):
+    """Float32 port of the reference kernel (pp_ref_f32.c, the CPU
+    baseline). Returns (hits sorted by (source, photon id), n_ph_sim,
+    photon steps executed)."""
+    hits = np.zeros(capacity, dtype=HIT_DTYPE)
+    sp, keep = _spec_struct(mono_wl, table, mapping)
+    nsim, nstep = c_int64(), c_int64()
+    n = lib().ppf_propagate(ctypes.cast(sources, c_void_p), len(sources), ctypes.byref(medium),
+                            ctypes.byref(sp), ctypes.cast(targets, c_void_p), len(targets),
+                            ctypes.c_uint64(seed), int(n_steps), float(photon_scaling),
+                            hits.ctypes.data_as(c_void_p), capacity, ctypes.byref(nsim),
+                            ctypes.byref(nstep), int(n_threads))
+    del keep
+    if n == -2:
+        raise ValueError("the Float32 port does not cover this source, medium or target kind")
+    if n < 0:
+        raise OverflowError("hit capacity exceeded")
+    h = hits[:n]
+    h = h[np.lexsort((h["photon_id"], h["source_ix"]))].copy()
+    return h, nsim.value, nstep.value
 
 
 # ---- surrogate MLP (pp_mlp_oracle.c) ----

@@ -709,6 +709,7 @@ int ppo_eval_fn(int fn, const pp_medium* m, const ppo_spectrum* sp, const float*
             y[0] = ppo_spectrum_sample(sp, x[0]);
             break;
         case PP_FN_SL_COS: y[0] = ppo_sl_cos(x[0], x[1]); break;
+        case PP_FN_MIXED_HGES: y[0] = ppo_mixed_hges_cos(x[0], x[1], x[2], x[3], x[4]); break;
         default: return -1;
         }
     }

@@ -86,6 +86,15 @@ int64_t ppo_propagate(const pp_source* sources, int n_sources, const pp_medium* 
                       uint64_t seed, int32_t n_steps, float photon_scaling, ppo_hit* hits,
                       int64_t capacity, int64_t* n_ph_sim, int n_threads);
 
+/* Float32 port of the reference kernel (pp_ref_f32.c, the CPU baseline):
+ * same arguments; hits unordered; *n_steps_done = photon steps executed;
+ * returns -2 for a source, medium or target kind the port does not cover. */
+int64_t ppf_propagate(const pp_source* sources, int n_sources, const pp_medium* m,
+                      const ppo_spectrum* sp, const pp_target* targets, int n_targets,
+                      uint64_t seed, int32_t n_steps, float photon_scaling, ppo_hit* hits,
+                      int64_t capacity, int64_t* n_ph_sim, int64_t* n_steps_done,
+                      int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

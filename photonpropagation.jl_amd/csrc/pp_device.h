@@ -353,8 +353,13 @@ __device__ __forceinline__ float normal_sample(const Stream& s, uint32_t sub) {
 }
 
 // Marsaglia-Tsang gamma(shape, scale). Each sampler block: a Box-Muller pair
-// (angle 2 pi u, sincos_2pi) and two acceptance uniforms (two attempts). shape < 1 boosts Gamma(shape+1)
-// by U^(1/shape), U from the sub-stream's boost block.
+// (angle 2 pi u, sincos_2pi) and two acceptance uniforms (two attempts).
+// shape < 1 boosts Gamma(shape+1) by U^(1/shape), U from the sub-stream's
+// boost block. The accepted value d (1 + c x)^3 scale boost amplifies the
+// relative error of 1 + c x three times, so that factor is formed with one
+// FMA from the float variate and the cube and the product are taken in FP64
+// (a handful of FP64 operations once per photon) and rounded once: a draw
+// whose decisions agree with an exact evaluation is then within ~2 ulp of it.
 __device__ inline float gamma_sample(float shape, float scale, const Stream& s, uint32_t sub) {
     float boost = 1.0f;
     if (shape < 1.0f) {
@@ -362,7 +367,8 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
         boost = powf(u01_open0(r.x), 1.0f / shape);
         shape += 1.0f;
     }
-    const float d = shape - 1.0f / 3.0f;
+    const double dd = (double)shape - 1.0 / 3.0;
+    const float d = (float)dd;
     const float c = 1.0f / sqrtf(9.0f * d);
     for (uint32_t i = 0; i < PP_GAMMA_MAX_BLOCKS; ++i) {
         const uint4 r = s.block(sampler_word(sub, i));
@@ -374,16 +380,17 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const float x = z[k];
-            float v = 1.0f + c * x;
-            if (v <= 0.0f) continue;
-            v = v * v * v;
+            const double v1 = fma((double)c, (double)x, 1.0);
+            if (v1 <= 0.0) continue;
+            const double v3 = v1 * v1 * v1;
+            const float v = (float)v3;
             const float u = ua[k];
             const float x2 = x * x;
             if (u < 1.0f - 0.0331f * (x2 * x2) || logf(u) < 0.5f * x2 + d * (1.0f - v + logf(v)))
-                return d * v * scale * boost;
+                return (float)(dd * v3 * (double)scale * (double)boost);
         }
     }
-    return d * scale * boost;
+    return (float)(dd * (double)scale * (double)boost);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,12 +399,15 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
 __device__ __forceinline__ float spectrum_sample(const float* tab, int n, int mapping, float mono,
                                                  float u) {
     if (n == 0) return mono;
-    float x = mapping == PP_SPEC_TEXTURE ? u * (float)n - 0.5f : u * (float)(n - 1);
+    // separately rounded products and sums (no contraction): the knot
+    // position and a + f (b - a) as the reference evaluates them
+    float x = mapping == PP_SPEC_TEXTURE ? __fadd_rn(__fmul_rn(u, (float)n), -0.5f)
+                                         : __fmul_rn(u, (float)(n - 1));
     x = fminf(fmaxf(x, 0.0f), (float)(n - 1));
     const int i = min((int)x, n - 2);
     const float f = x - (float)i;
     const float a = tab[i], b = tab[i + 1];
-    return a + f * (b - a);
+    return __fadd_rn(a, __fmul_rn(f, b - a));
 }
 
 }  // namespace pp

@@ -537,6 +537,12 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         y[0] = pp::sl_cos(ph, x[1]);
         break;
     }
+    case PP_FN_MIXED_HGES: {
+        // the sampler phase_cos runs for a MixedHGES medium in the kernel
+        pp::Phase ph = pp::make_phase(PP_PHASE_MIXED_HG_ES, x[0], x[1], x[2]);
+        y[0] = pp::phase_cos(ph, x[3], x[4]);
+        break;
+    }
     case PP_FN_PLANE_ISEC: {
         float d;
         const uint32_t kind = bits(x[0]);
@@ -1570,8 +1576,8 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
 
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, int out_stride,
                int64_t n) {
-    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1, 2, 13};
-    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1, 1, 2};
+    static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1, 2, 13, 5};
+    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1, 1, 2, 1};
     if (!ctx) return fail(PP_ERR_INVALID, "pp_eval_fn: NULL ctx");
     if (fn < 0 || fn >= PP_FN_COUNT) return fail(PP_ERR_INVALID, "pp_eval_fn: unknown function");
     if (n < 0 || (n > 0 && (!in || !out)))

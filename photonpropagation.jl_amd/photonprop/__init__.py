@@ -7,9 +7,10 @@ setup, emitter, target, medium and spectrum types and ``propagate_photons``
 with the reference's hit-table layout.
 """
 from . import _lib, autotune, workloads
-from .detection import (Circular, HomogeneousDetector, PhotonTarget, Rectangular, Spherical,
-                        circular_target, make_detector_grid, rectangular_target,
-                        spherical_target, targets_to_pod)
+from .detection import (Circular, DetectorLine, DetectorLines, HomogeneousDetector, PhotonTarget,
+                        Rectangular, Spherical, circular_target, flatten_targets,
+                        make_detector_grid, rectangular_target, spherical_target,
+                        targets_to_pod)
 from .lightyield import (AxiconeEmitter, CherenkovEmitter, CherenkovTrackEmitter,
                          CollimatedIsotropicEmitter, ExtendedCherenkovEmitter,
                          ExtendedCherenkovEmitterCustomSmear, ExtendedCherenkovEmitterNoSmear,
@@ -22,8 +23,8 @@ from .medium import (CascadiaMediumProperties, HomogenousIceProperties, MediumPr
                      SimpleMediumProperties, make_homogenous_clearice_properties,
                      make_simple_medium)
 from .processing import (DeviceHitBuffer, HostHitBuffer, PhotonPropOOMException,
-                         calc_total_weight, hits_to_dataframe, make_hit_buffers,
-                         propagate_photons, run_photon_prop_no_local_cache)
+                         assemble_hit_table, calc_total_weight, hits_to_dataframe,
+                         make_hit_buffers, propagate_photons, run_photon_prop_no_local_cache)
 from .propsetup import PhotonPropSetup
 from .spectrum import (DeviceSpectralDist, InterpolatedSpectralDist, Monochromatic, Spectrum,
                        frank_tamm, integrate_gauss_quad, make_biased_cherenkov_spectrum,

@@ -76,8 +76,40 @@ def circular_target(position, radius, module_id):
                         int(module_id))
 
 
+@dataclass
+class DetectorLine:
+    """DetectorLine(x_pos, y_pos, radius, targets) (src/photon_prop_cuda.jl:
+    418-423): one string of target shapes; x_pos, y_pos and radius bound the
+    string for the reference's own two-level line search."""
+    x_pos: float
+    y_pos: float
+    radius: float
+    targets: list
+
+
+@dataclass
+class DetectorLines:
+    """DetectorLines(lines) (src/photon_prop_cuda.jl:425-427)."""
+    lines: list
+
+
+def flatten_targets(targets):
+    """The target list the library searches. A DetectorLines becomes the
+    shapes of its lines in line order, so target k (1-based) is shape t of
+    line l with k = (l - 1) * n + t for lines of n shapes. The grid search
+    replaces the reference's line kernel (src/photon_prop_cuda.jl:552-694),
+    whose quirks are not reproduced: there every line is a separate 2-D test,
+    only the first line hit is searched and module_ix = t + n * l counts
+    from the second line on."""
+    if isinstance(targets, DetectorLines):
+        return [shape for line in targets.lines for shape in line.targets]
+    return list(targets)
+
+
 def targets_to_pod(targets):
-    """Lower a target list to the pp_target array (index order preserved)."""
+    """Lower a target list (or a DetectorLines) to the pp_target array
+    (index order preserved)."""
+    targets = flatten_targets(targets)
     arr = (_lib.pp_target * max(len(targets), 1))()
     for i, t in enumerate(targets):
         shape = t.shape if hasattr(t, "shape") else t

@@ -17,7 +17,7 @@ import numpy as np
 import pandas as pd
 
 from . import _lib
-from .detection import targets_to_pod
+from .detection import flatten_targets, targets_to_pod
 from .spectrum import DeviceSpectralDist, Monochromatic
 
 HIT_COLUMNS = ("position", "direction", "initial_direction", "wavelength", "time",
@@ -114,8 +114,9 @@ def stage(dev: DeviceHitBuffer, targets, medium, spectrum):
     mono, tab, n, mapping = _spectrum_args(spectrum)
     tp = None if tab is None else tab.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     _lib.check(lib.pp_set_spectrum(dev.handle, mono, tp, n, mapping), "pp_set_spectrum")
-    pods = targets_to_pod(targets)
-    _lib.check(lib.pp_set_targets(dev.handle, pods, len(targets)), "pp_set_targets")
+    flat = flatten_targets(targets)
+    pods = targets_to_pod(flat)
+    _lib.check(lib.pp_set_targets(dev.handle, pods, len(flat)), "pp_set_targets")
 
 
 def sources_to_pod(sources):
@@ -166,15 +167,18 @@ def run_photon_prop_no_local_cache(sources, targets, medium, spectrum, seed,
 
 
 def calc_total_weight(df: pd.DataFrame, setup) -> pd.DataFrame:
-    """src/processing.jl:32-44: abs_weight = exp(-dist/abs_len(wl)) (float64),
-    total_weight = base_weight * abs_weight."""
+    """src/processing.jl:32-44: abs_weight = exp(-Float64(dist ./ abs_len(wl))),
+    total_weight = base_weight * abs_weight. The medium is Float32, so the
+    absorption length and the ratio dist ./ abs_len are Float32 values and
+    only the exponential is taken in Float64, as in the reference. An empty
+    table is returned unchanged (no weight columns are added)."""
     if len(df) == 0:
         return df
     abs_len = setup.medium.absorption_length(df["wavelength"].to_numpy(np.float64))
-    df["abs_weight"] = np.exp(-(df["dist_travelled"].to_numpy(np.float64) / abs_len))
+    ratio = df["dist_travelled"].to_numpy(np.float32) / abs_len.astype(np.float32)
+    df["abs_weight"] = np.exp(-ratio.astype(np.float64))
     df["total_weight"] = df["base_weight"].to_numpy() * df["abs_weight"].to_numpy()
     return df
-
 
 
 def hits_to_dataframe(cols, vector_columns=True):
@@ -225,14 +229,34 @@ def propagate_photons(setup, hit_buffer_cpu=None, hit_buffer_gpu=None, steps=15,
     finally:
         if own:
             hit_buffer_gpu.close()
+    return assemble_hit_table(hits, n_ph_sim, setup, vector_columns)
+
+
+def base_weight(total_photons: int, n_ph_sim: int) -> float:
+    """sum(source.photons) / n_ph_sim of src/processing.jl:70: Julia's Int64 /
+    Int64 is a Float64 division, so a run that simulated no photon gives
+    Inf (NaN if the sources hold none)."""
+    num, den = float(total_photons), float(n_ph_sim)
+    if den == 0.0:
+        return float("nan") if num == 0.0 else float("inf")
+    return num / den
+
+
+def assemble_hit_table(hits, n_ph_sim, setup, vector_columns=True) -> pd.DataFrame:
+    """The host part of propagate_photons after the propagation
+    (src/processing.jl:67-72, :32-44): the hit columns in the PhotonHit
+    order, module_id remapped from the 1-based target index to the target's
+    module_id (mod_ids[df.module_id]), base_weight = sum(photons) / n_ph_sim,
+    then calc_total_weight (abs_weight, total_weight)."""
     df = hits_to_dataframe(hits, vector_columns)
-    mod_ids = np.array([getattr(t, "module_id", i + 1) for i, t in enumerate(setup.targets)],
-                       dtype=np.int64)
+    # bare shapes (and the shapes of a DetectorLines) keep their 1-based index
+    mod_ids = np.array([getattr(t, "module_id", i + 1)
+                        for i, t in enumerate(flatten_targets(setup.targets))], dtype=np.int64)
     if len(df):
-        df["module_id"] = mod_ids[df["module_id"].to_numpy() - 1]
-    total = float(sum(int(s.photons) for s in setup.sources))
-    df["base_weight"] = total / n_ph_sim if n_ph_sim > 0 else np.nan
-    df["abs_weight"] = np.zeros(len(df), dtype=np.float64)
-    df["total_weight"] = np.zeros(len(df), dtype=np.float64)
+        ix = df["module_id"].to_numpy()
+        if ix.min() < 1 or ix.max() > len(mod_ids):
+            raise IndexError("hit module index outside the target list")
+        df["module_id"] = mod_ids[ix - 1]
+    df["base_weight"] = base_weight(sum(int(s.photons) for s in setup.sources), n_ph_sim)
     calc_total_weight(df, setup)
     return df

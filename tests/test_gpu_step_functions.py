@@ -6,7 +6,9 @@ of the Float32 output). Tolerance: 4 FP32 ulp of the output's scale plus the
 function's own conditioning, i.e. the largest change of the exact output
 when any one Float32 input moves by 1 ulp (kappa). Unit-vector components
 are measured at scale 1 (ulp(1) = 2^-23); scalars at their magnitude.
-Philox and the spectrum lookup are bit-exact."""
+Philox and the spectrum lookup are bit-exact. Rejection samplers (gamma)
+are compared draw by draw; decisions flipped by a near-tie are counted
+apart from the 4-ulp bound."""
 import numpy as np
 import pytest
 
@@ -256,19 +258,48 @@ def test_cherenkov_track_direction(P, O, dev):
 
 
 def test_gamma_sampler_same_draws(P, O, dev):
-    n = 50_000
-    rows = np.zeros((n, 7), dtype=np.float32)
-    rows[:, 0] = 7.837
-    rows[:, 1] = 1 / 0.632
-    w = rows.view(np.uint32)
-    w[:, 2] = 99
-    w[:, 4] = np.arange(n)
-    g, r = both(P, O, dev, "GAMMA", rows)
-    err = np.abs(g[:, 0] - r[:, 0]) / ulp(r[:, 0])
-    report("gamma", err)
-    # identical accept/reject decisions give values within a few ulp; a flipped
-    # decision (boundary case) gives a different draw
-    assert np.mean(err <= 8) > 0.9999
+    """Marsaglia-Tsang draws from the same Philox blocks. Where the GPU and
+    the oracle take the same accept/reject decisions the draw is the same
+    Box-Muller variate and agrees within 4 ulp; a decision flipped by the last
+    bits of a near-tie picks another variate, which shows as a relative
+    difference far above rounding. Flips are counted separately and must
+    stay rare."""
+    for shape, scale in ((7.837, 1 / 0.632), (1.776, 1.0), (0.164, 1.0)):
+        n = 50_000
+        rows = np.zeros((n, 7), dtype=np.float32)
+        rows[:, 0] = shape
+        rows[:, 1] = scale
+        w = rows.view(np.uint32)
+        w[:, 2] = 99
+        w[:, 4] = np.arange(n)
+        g, r = both(P, O, dev, "GAMMA", rows)
+        rel = np.abs(g[:, 0] - r[:, 0]) / np.maximum(np.abs(r[:, 0]), 1e-37)
+        flip = rel > 1e-5
+        err = np.abs(g[~flip, 0] - r[~flip, 0]) / ulp(r[~flip, 0])
+        report(f"gamma k={shape}", err)
+        print(f"gamma k={shape}: {flip.sum()} of {n} draws with a flipped decision")
+        assert err.max() <= 4
+        assert flip.sum() <= 5
+
+
+@pytest.mark.parametrize("g_", [0.95, 0.5])
+def test_mixed_hges_kernel_sampler(P, O, dev, g_):
+    """The branch-free HG/ES mixture the propagation kernel calls (both
+    samplers share one Newton reciprocal) against the oracle's MixedHGES
+    (branch choice, then the cosine)."""
+    n = 200_000
+    x = np.zeros((n, 5), np.float32)
+    x[:, 0] = g_
+    x[:, 1] = 0.835
+    x[:, 2] = 0.8
+    x[:, 3] = RNG.random(n, dtype=np.float32)
+    x[:, 4] = RNG.random(n, dtype=np.float32)
+    g, r = check_vec(P, O, dev, "MIXED_HGES", x, f"mixed_hges g={g_}")
+    hg = x[:, 3] < x[:, 2]
+    assert 0.79 < hg.mean() < 0.81
+    # the ES branch has E[cos] = 0, the HG branch E[cos] = g
+    assert abs(g[~hg, 0].mean()) < 0.01
+    assert abs(g[hg, 0].mean() - g_) < 0.01
 
 
 def test_spectrum_lookup_bit_exact(P, O, dev):
@@ -278,8 +309,7 @@ def test_spectrum_lookup_bit_exact(P, O, dev):
         spec = P.DeviceSpectralDist(tab.table, tab.normalization, mapping)
         P.processing.stage(dev, [P.spherical_target((0, 0, 0), 1.0, 1)], m, spec)
         u = RNG.random((100_000, 1), dtype=np.float32)
+        u[:4, 0] = [0.0, 1.0 - 2.0**-24, 0.5, 2.0**-24]
         g = P._lib.eval_fn(dev, "SPECTRUM", u)
         r = O.eval_fn("SPECTRUM", u, spectrum=(0.0, tab.table, mapping))
-        err = np.abs(g - r) / ulp(r)
-        report(f"spectrum mapping={mapping}", err)
-        assert err.max() <= 1
+        np.testing.assert_array_equal(g.view(np.uint32), r.view(np.uint32))

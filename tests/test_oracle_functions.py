@@ -66,6 +66,30 @@ def test_rot_from_ez_invariants(O):
     np.testing.assert_allclose(rot(O, a[ok], k), k, atol=5e-6)
 
 
+def test_rotation_convention_does_not_change_the_distribution(O):
+    """rot_from_ez_fast's azimuth convention is parity unpinned (PhysicsTools
+    0.8.2 is absent): the oracle uses the minimal rotation, another reading is
+    R = Rz(phi_a) Ry(theta_a). Two rotations that both take e_z to a differ
+    by a turn about e_z, R2 = R1 Rz(psi), and a direction drawn with a uniform
+    azimuth is invariant under that turn. So the conventions give different
+    directions for the same uniforms but the same distribution, and the
+    propagation statistics do not depend on the choice."""
+    n = 100_000
+    a = np.tile(np.float32([[0.48, -0.6, 0.64]]), (n, 1))
+    ct = np.float32(0.8)
+    phi = RNG.random(n) * 2 * np.pi
+    v = np.stack([np.cos(phi) * 0.6, np.sin(phi) * 0.6, np.full(n, ct)], 1).astype(np.float32)
+    r_min = rot(O, a, v)
+    th, ph = np.arccos(0.64), np.arctan2(-0.6, 0.48)
+    ry = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+    rz = np.array([[np.cos(ph), -np.sin(ph), 0], [np.sin(ph), np.cos(ph), 0], [0, 0, 1]])
+    r_zy = v.astype(np.float64) @ (rz @ ry).T
+    np.testing.assert_allclose(rz @ ry @ [0, 0, 1], a[0], atol=1e-6)
+    assert np.median(np.linalg.norm(r_min - r_zy, axis=1)) > 0.3  # different per draw
+    for k in range(3):
+        assert stats.ks_2samp(r_min[:, k], r_zy[:, k]).pvalue > 1e-3
+
+
 def test_rot_from_ez_poles(O):
     b = np.float32([[0.3, -0.4, 0.866025]])
     np.testing.assert_allclose(rot(O, np.float32([[0, 0, 1]]), b), b, atol=1e-7)
@@ -202,8 +226,62 @@ def test_cascadia_reference_values(O, P):
     assert abs(med.scattering_length(488.0) - ref["scattering_length_m"]) < 1e-4
 
 
+# Refractive index of pure water at 0.101325 MPa (sodium D line 589.3 nm at
+# 0-30 C; hydrogen F and C lines and mercury 435.8 / 546.1 nm at 20 C):
+# CRC Handbook of Chemistry and Physics / IAPWS (1997) values.
+PUBLISHED_WATER_INDEX = [  # (wavelength nm, temperature C, n)
+    (589.3, 0.0, 1.33395), (589.3, 10.0, 1.33369), (589.3, 20.0, 1.33299),
+    (589.3, 25.0, 1.33250), (589.3, 30.0, 1.33194),
+    (486.13, 20.0, 1.33712), (656.27, 20.0, 1.33114), (435.83, 20.0, 1.34020),
+    (546.07, 20.0, 1.33447)]
+
+
+def test_quan_fry_against_published_water_indices(O, P):
+    """Quan & Fry (1995) reduces, for pure water (S = 0) at atmospheric
+    pressure, to a fit of the measured index over 400-700 nm and 0-30 C. The
+    oracle's phase index (and the host formula the library lowers) must
+    reproduce the published values within 4e-5, which a wrong value or sign
+    of n0, n4, n5, n8 or n9 fails (each such slip moves n by >= 5e-5). The
+    salinity coefficients are checked only by the size of the seawater
+    increment (n(S=35) - n(S=0) = 0.0064 at 20 C, 589 nm); they and the
+    pressure term stay parity unpinned."""
+    for wl, t, ref in PUBLISHED_WATER_INDEX:
+        med = P.CascadiaMediumProperties(salinity=0.0, temperature=t, pressure=0.0)
+        n_or = float(O.eval_fn("CASCADIA", [[wl]], medium=med.to_pod())[0, 2])
+        n_host = float(med.phase_refractive_index(wl))
+        assert abs(n_or - ref) < 4e-5, (wl, t, n_or, ref)
+        assert abs(n_host - ref) < 4e-5, (wl, t, n_host, ref)
+    sea = P.CascadiaMediumProperties(salinity=35.0, temperature=20.0, pressure=0.0)
+    fresh = P.CascadiaMediumProperties(salinity=0.0, temperature=20.0, pressure=0.0)
+    dn = float(sea.phase_refractive_index(589.3) - fresh.phase_refractive_index(589.3))
+    assert 0.0060 < dn < 0.0068
+
+
+# Density of liquid water at 0.101325 MPa, kg/m^3 (IAPWS-95).
+MEASURED_WATER_DENSITY = [(0.01, 999.84), (4.0, 999.97), (10.0, 999.70), (20.0, 998.21),
+                          (30.0, 995.65), (40.0, 992.22), (60.0, 983.20), (80.0, 971.80),
+                          (99.9, 958.37)]
+
+
+def test_dippr105_water_density(P):
+    """CASCADIA_DENSITY = DIPPR105(T) (src/water_properties.jl:38) sets the
+    radiation length in metres of the cascade profile (src/lightyield.jl:
+    192). The correlation rho = A / B^(1 + (1 - T/C)^D) with the Dortmund
+    Data Bank's water coefficients is a fit of the liquid range: it must
+    agree with measured densities within 1.5% over 0-100 C (a wrong
+    coefficient or exponent misses by far more). Which coefficients
+    CherenkovMediumBase uses is parity unpinned."""
+    for t, rho in MEASURED_WATER_DENSITY:
+        got = P.medium.dippr105(t + 273.15)
+        assert abs(got / rho - 1.0) < 0.015, (t, got, rho)
+    # the value the Cascadia medium uses (1.8 C), and its radiation length
+    assert 1010.0 < P.medium.CASCADIA_DENSITY < 1015.0
+    lrad = P.medium.CASCADIA_RADIATION_LENGTH / P.medium.CASCADIA_DENSITY * 10.0
+    assert abs(lrad - 0.3561) < 5e-4
+
+
 def test_quan_fry_invariants(O, P):
-    """Parity unpinned (no reference value); physical invariants only."""
+    """Physical invariants of the Cascadia (sea water, 265.9 atm) index."""
     wl = np.linspace(300, 800, 101, dtype=np.float32)
     m = P.CascadiaMediumProperties().to_pod()
     out = O.eval_fn("CASCADIA", wl[:, None], medium=m).astype(np.float64)
