@@ -221,7 +221,48 @@ def pmt_leg():
             "detected_frac": det / n, "bytes_per_hit": bytes_per_hit,
             "achieved_gbps": gbps, "hbm_frac": gbps / 8000.0,
             "note": "16 PMTs per hit: acos, a square root, exp and a reciprocal each, one "
-                    "Philox block; vector-ALU bound, not HBM bound"}
+                    "Philox block; vector-ALU bound, not HBM bound",
+            "end_to_end": pmt_end_to_end()}
+
+
+def pmt_end_to_end():
+    """Propagation into POMs, then PMT detection of the hits on the device
+    (pp_pom_pmt_hits_ctx reads the context's hit buffer; only the PMT ids come
+    back): the cascade event with its 320 modules as POMs, 1e8 photons. Wall
+    clock of pp_propagate alone and of pp_propagate + detection, median of
+    three after one warm-up."""
+    import ctypes
+
+    import numpy as np
+
+    import photonprop as P
+    from photonprop import pom
+    lib = P._lib.load_library()
+    wl = P.workloads.cascade_16_strings()
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 0
+    _, dev = P.make_hit_buffers(capacity=1 << 24, config=cfg)
+    pods = P.processing.sources_to_pod(wl.sources)
+    nh, ns = ctypes.c_int64(), ctypes.c_int64()
+    with dev:
+        P.processing.stage(dev, wl.targets, wl.medium, wl.spectrum)
+        prop, both, kern = [], [], []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            P._lib.check(lib.pp_propagate(dev.handle, pods, 1, ctypes.c_uint64(wl.seed), 15, 1.0,
+                                          ctypes.byref(nh), ctypes.byref(ns)), "pp_propagate")
+            t1 = time.perf_counter()
+            dev.n_hits = nh.value
+            t = {}
+            ids = pom.pmt_ids_in_context(dev, seed=3, timing=t)
+            t2 = time.perf_counter()
+            prop.append(t1 - t0)
+            both.append(t2 - t0)
+            kern.append(t["kernel_ms"])
+    med = lambda v: sorted(v[1:])[1]  # noqa: E731
+    return {"hits": int(nh.value), "detected": int(np.sum(ids > 0)),
+            "propagate_ms": med(prop) * 1e3, "propagate_plus_pmt_ms": med(both) * 1e3,
+            "pmt_kernel_ms": med(kern), "photons_per_s_incl_pmt": ns.value / med(both)}
 
 
 def copy_leg(lib, P, dev, pods, wl):
@@ -404,6 +445,7 @@ def surrogate_leg(args):
         "mean_mc_stat_err": res["mean_mc_stat_err"],
         "time_ll_per_hit": {"surrogate": res.get("time_ll_surrogate"),
                             "mc_histogram": res.get("time_ll_mc_hist")},
+        "strata": res["strata"],
         "judge": f"{args.sur_events} held-out cascades (1-1000 TeV) in the 1000-module array; "
                  "pairs with >= 2500 effective MC hits (<= 2% MC error); MC at min(yield, 1e9) "
                  "photons per event, rate = pairs / MC kernel time",

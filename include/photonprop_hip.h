@@ -288,6 +288,13 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
  * kernels of the last pp_propagate, and the number of kernel launches. */
 int pp_last_timing(pp_ctx* ctx, float* kernel_ms, int32_t* n_launches);
 
+/* Frees the buffers a context grows on demand and keeps between calls: the
+ * device buffer of the ordered copy (about 88 bytes per hit), the pinned
+ * staging of pp_copy_hits (at most 64 MB) and the device buffers of
+ * pp_pom_pmt_hits_ctx. They are allocated again by the next call that needs
+ * them. No counterpart in the reference (its buffers are the caller's). */
+int pp_release_scratch(pp_ctx* ctx);
+
 /* The context's hipStream_t, as an opaque pointer. */
 void* pp_stream(pp_ctx* ctx);
 
@@ -339,6 +346,21 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* model, const float* wl_grid,
                     const int32_t* source_ix, uint64_t seed, int32_t* pmt_id_out,
                     int64_t* n_detected, float* kernel_ms);
 
+/* The same detection for the hits of the last pp_propagate of `ctx`, read
+ * from the context's device hit buffer on its stream (no host round trip of
+ * the hits): hits [first, first + n) in the order pp_copy_hits returns them;
+ * each hit's module centre and radius come from the context's target list,
+ * which must hold spheres. pmt_id_out (host, n int32) receives the PMT ids;
+ * *n_detected (may be NULL) counts the detected hits of the range; *kernel_ms
+ * (may be NULL) the detection kernel's device time. The context keeps the
+ * device buffers (acceptance table, 4 bytes per hit), grown when a call needs
+ * more; a call of a size already seen allocates nothing. Replaces the
+ * per-module check_pmt_hit loop of make_hits_from_photons
+ * (src/processing.jl:88-144, src/pom.jl:246-305). */
+int pp_pom_pmt_hits_ctx(pp_ctx* ctx, const pp_pom_model* model, const float* wl_grid,
+                        const float* acc1, const float* acc2, uint64_t seed, int64_t first,
+                        int64_t n, int32_t* pmt_id_out, int64_t* n_detected, float* kernel_ms);
+
 /* Autotuner. The reference fixes its launch shape from the occupancy API
  * (src/photon_prop_cuda.jl:914-920); this library searches it instead.
  * Times pp_propagate on a sample of `sources` (photon counts scaled so that
@@ -381,7 +403,8 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *   PP_FN_CASCADIA        in (wl)                out (sca_len, c_grp, n_phase)
  *   PP_FN_TRACK_DIR       in (u1,u2)             out (x,y,z)
  *   PP_FN_PHILOX          in (c0..c3,k0,k1 as u32 bits) out (r0..r3 as u32 bits)
- *   PP_FN_GAMMA           in (k, theta, key_lo, key_hi, pid_lo, src as u32 bits) out (x)
+ *   PP_FN_GAMMA           in (k, theta, key_lo, key_hi, pid_lo, pid_hi, src as u32 bits)
+ *                         out (draw, accepted normal variate)
  *   PP_FN_SPECTRUM        in (u)                 out (wl)  (uses the context's spectrum)
  *   PP_FN_SL_COS          in (g, u)              out (cos)  simplified Liu
  *   PP_FN_PLANE_ISEC      in (kind, cx,cy,cz, a, b, px,py,pz, dx,dy,dz, step) out (hit,d)

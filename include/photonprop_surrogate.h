@@ -127,6 +127,19 @@ int pp_sur_eval_pairs(pp_mlp* m, const pp_cascade* events, int n_events, const f
                       int n_modules, const pp_sur_domain* dom, float* expected_hits,
                       float* mixture, uint8_t* fallback, float* kernel_ms);
 
+/* Training labels of the samples of one propagation (the sources of the
+ * last pp_propagate of `ctx`, one sample each), reduced on the device from the
+ * context's hit records: labels (host, n_sources x (1 + PP_SUR_BINS)) gets
+ * per source the absorption-weighted hit sum w and the weighted histogram of
+ * x = t - tgeo[source] + PP_SUR_TIME_SHIFT over PP_SUR_BINS bins of
+ * [0, PP_SUR_TMAX) ns. The weight of a hit is exp(-dist / (abs_scale *
+ * abs_len(wl))), abs_len linear in the table (abs_wl, abs_len: n_abs knots,
+ * clamped at its ends). Sums are exact to 2^-32 and independent of the hit
+ * order. */
+typedef struct pp_ctx pp_ctx;
+int pp_sur_hit_labels(pp_ctx* ctx, const double* abs_wl, const double* abs_len, int n_abs,
+                      double abs_scale, const double* tgeo, int n_sources, float* labels);
+
 #ifdef __cplusplus
 }
 #endif

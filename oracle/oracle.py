@@ -21,7 +21,7 @@ FN = dict(ISOTROPIC_DIR=0, ROT_FROM_EZ=1, HG_COS=2, ES_COS=3, UPDATE_DIR=4, SPHE
           STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11, SL_COS=12,
           PLANE_ISEC=13, MIXED_HGES=14)
 FN_ARITY = {0: (2, 3), 1: (6, 3), 2: (2, 1), 3: (2, 1), 4: (5, 3), 5: (11, 2), 6: (2, 1),
-            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1), 12: (2, 1),
+            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 2), 11: (1, 1), 12: (2, 1),
             13: (13, 2), 14: (5, 1)}
 
 HIT_DTYPE = np.dtype([("pos", np.float32, 3), ("dir", np.float32, 3),

@@ -346,7 +346,16 @@ static float ppo_normal(const ppo_stream* s, uint32_t sub) {
  * uniforms; shape < 1 uses Gamma(shape+1) * U^(1/shape) with U from the
  * sub-stream's boost block. After PP_GAMMA_MAX_BLOCKS blocks without
  * acceptance (probability < 1e-100 for shape >= 0.1) returns d*scale. */
+static float ppo_gamma_x(float shape, float scale, const ppo_stream* s, uint32_t sub,
+                         float* x_acc);
 float ppo_gamma(float shape, float scale, const ppo_stream* s, uint32_t sub) {
+    return ppo_gamma_x(shape, scale, s, sub, NULL);
+}
+
+/* as ppo_gamma; *x_acc (may be NULL) receives the accepted normal variate
+ * rounded to float (0 if none) */
+static float ppo_gamma_x(float shape, float scale, const ppo_stream* s, uint32_t sub,
+                         float* x_acc) {
     double boost = 1.0, k = shape;
     if (shape < 1.0f) {
         uint32_t r[4];
@@ -370,10 +379,13 @@ float ppo_gamma(float shape, float scale, const ppo_stream* s, uint32_t sub) {
             if (v <= 0.0) continue;
             v = v * v * v;
             double u = ua[j];
-            if (u < 1.0 - 0.0331 * (x * x) * (x * x) || log(u) < 0.5 * x * x + d * (1.0 - v + log(v)))
+            if (u < 1.0 - 0.0331 * (x * x) * (x * x) || log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) {
+                if (x_acc) *x_acc = (float)x;
                 return (float)(d * v * scale * boost);
+            }
         }
     }
+    if (x_acc) *x_acc = 0.0f;
     return (float)(d * scale * boost);
 }
 
@@ -699,9 +711,9 @@ int ppo_eval_fn(int fn, const pp_medium* m, const ppo_spectrum* sp, const float*
         case PP_FN_GAMMA: {
             uint32_t w[5];
             memcpy(w, x + 2, 20);
-            uint64_t seed = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
-            uint64_t pid = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
-            y[0] = ppo_gamma_sample(x[0], x[1], seed, pid, w[4]);
+            uint32_t key[2] = {w[0], w[1]};
+            ppo_stream st = {key, (uint64_t)w[2] | ((uint64_t)w[3] << 32), w[4]};
+            y[0] = ppo_gamma_x(x[0], x[1], &st, PP_SUB_LONG_GAMMA, &y[1]);
             break;
         }
         case PP_FN_SPECTRUM:

@@ -14,6 +14,13 @@
 
 #include "../../include/photonprop_hip.h"
 
+// No contraction in any translation unit that includes this header: each
+// fused multiply-add is written as fmaf and every other product and sum is
+// rounded on its own. A function then gives the same bits wherever it is
+// inlined (the propagation kernels inline the step functions into several
+// schedules whose results must agree bit for bit).
+#pragma clang fp contract(off)
+
 namespace pp {
 
 constexpr float kTwoPi = 6.28318548202514648438f;  // float(2*pi), as T(2*pi) in Julia
@@ -127,14 +134,14 @@ __device__ __forceinline__ f3 isotropic_dir(float u1, float u2) {
 // no cancellation on either side). a = -e_z exactly turns by pi about x.
 __device__ __forceinline__ f3 rot_from_ez(f3 a, f3 b) {
     const float c = a.z;
-    const float s2 = a.x * a.x + a.y * a.y;
+    const float s2 = fmaf(a.x, a.x, a.y * a.y);
     const bool up = c >= 0.0f;
     const float h = (up ? 1.0f : (1.0f - c)) * rcp_nr(up ? (1.0f + c) : s2);
-    const float vdb = a.x * b.y - a.y * b.x;
+    const float hv = h * fmaf(a.x, b.y, -(a.y * b.x));  // h (v.b)
     f3 o;
-    o.x = c * b.x + a.x * b.z - h * a.y * vdb;
-    o.y = c * b.y + a.y * b.z + h * a.x * vdb;
-    o.z = c * b.z - (a.x * b.x + a.y * b.y);
+    o.x = fmaf(-a.y, hv, fmaf(c, b.x, a.x * b.z));
+    o.y = fmaf(a.x, hv, fmaf(c, b.y, a.y * b.z));
+    o.z = fmaf(c, b.z, -fmaf(a.x, b.x, a.y * b.y));
     if (!up && s2 == 0.0f) o = make3(b.x, -b.y, -b.z);
     return o;
 }
@@ -157,8 +164,9 @@ __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg
     Phase p;
     p.kind = kind;
     p.hg_fraction = hg_fraction;
-    p.hg_A = (1.0f + g * g) / (2.0f * g);
-    p.hg_B = 1.0f - g * g;
+    // 1 +- g^2 with one rounding: 1 - g^2 cancels for g near 1
+    p.hg_A = fmaf(g, g, 1.0f) / (2.0f * g);
+    p.hg_B = fmaf(-g, g, 1.0f);
     p.hg_C = 1.0f / (2.0f * g);
     p.hg_D0 = 1.0f - g;
     p.hg_D1 = 2.0f * g;
@@ -172,21 +180,21 @@ __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg
 __device__ __forceinline__ float clamp1(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
 
 __device__ __forceinline__ float hg_cos(const Phase& p, float u) {
-    const float t = p.hg_B * rcp_nr(p.hg_D0 + p.hg_D1 * u);
-    return clamp1(p.hg_A - p.hg_C * (t * t));
+    const float t = p.hg_B * rcp_nr(fmaf(p.hg_D1, u, p.hg_D0));
+    return clamp1(fmaf(-p.hg_C, t * t, p.hg_A));
 }
 
 __device__ __forceinline__ float es_cos(const Phase& p, float u) {
-    const float q = p.es_Q * (1.0f - 2.0f * u);
+    const float q = p.es_Q * fmaf(-2.0f, u, 1.0f);
     const float hq = 0.5f * fabsf(q);
-    const float w = cbrt_pos(hq + sqrt_hw(hq * hq + p.es_P3cube));
-    const float x = w - p.es_P3 * rcp_nr(w);
+    const float w = cbrt_pos(hq + sqrt_hw(fmaf(hq, hq, p.es_P3cube)));
+    const float x = fmaf(-p.es_P3, rcp_nr(w), w);
     return clamp1(q > 0.0f ? -x : x);
 }
 
 // simplified Liu (PPC's ice phase function): inverse CDF 2 u^e - 1, e = (1-g)/(1+g)
 __device__ __forceinline__ float sl_cos(const Phase& p, float u) {
-    return clamp1(2.0f * exp2f(p.sl_exp * log2f(u)) - 1.0f);
+    return clamp1(fmaf(2.0f, exp2f(p.sl_exp * log2f(u)), -1.0f));
 }
 
 // The HG/ES mixture without a branch: nearly every wave holds photons of both
@@ -194,13 +202,13 @@ __device__ __forceinline__ float sl_cos(const Phase& p, float u) {
 // of HG's denominator or of ES's root w (the same values as hg_cos and es_cos).
 __device__ __forceinline__ float hg_es_cos(const Phase& p, float u_choice, float u) {
     const bool hg = u_choice < p.hg_fraction;
-    const float q = p.es_Q * (1.0f - 2.0f * u);
+    const float q = p.es_Q * fmaf(-2.0f, u, 1.0f);
     const float hq = 0.5f * fabsf(q);
-    const float w = cbrt_pos(hq + sqrt_hw(hq * hq + p.es_P3cube));
-    const float r = rcp_nr(hg ? p.hg_D0 + p.hg_D1 * u : w);
+    const float w = cbrt_pos(hq + sqrt_hw(fmaf(hq, hq, p.es_P3cube)));
+    const float r = rcp_nr(hg ? fmaf(p.hg_D1, u, p.hg_D0) : w);
     const float t = p.hg_B * r;
-    const float x = w - p.es_P3 * r;
-    return clamp1(hg ? p.hg_A - p.hg_C * (t * t) : (q > 0.0f ? -x : x));
+    const float x = fmaf(-p.es_P3, r, w);
+    return clamp1(hg ? fmaf(-p.hg_C, t * t, p.hg_A) : (q > 0.0f ? -x : x));
 }
 
 __device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float u) {
@@ -211,7 +219,7 @@ __device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float
 
 // update_direction   src/photon_prop_cuda.jl:280-298
 __device__ __forceinline__ f3 update_direction(f3 dir, float ct, float u_phi) {
-    const float st = sqrt_hw(1.0f - ct * ct);
+    const float st = sqrt_hw(fmaf(-ct, ct, 1.0f));
     return rot_from_ez(dir, dir_from_cos_phi(ct, st, u_phi));
 }
 
@@ -310,13 +318,21 @@ __device__ __forceinline__ MediumValues medium_values(const Medium& m, float wl)
         n = (float)nd;
         v.inv_c_grp = (float)(ng * (1.0 / (double)kCVacMNs));
     } else {
-        const float lx = log2f(wl * (1.0f / 550.0f));
-        const float b = m.kop1 * exp2f(-4.3f * lx) + m.kop2 * exp2f(-1.7f * lx) +
-                        m.kop3 * exp2f(-0.3f * lx);
-        v.sca_len = m.sca_scale / b;
+        // b = k1 x^-4.3 + k2 x^-1.7 + k3 x^-0.3, x = wl/550, with one
+        // transcendental t = x^-0.3 = q^0.3 (q = 1/x; the small exponent keeps
+        // log2f's error out of the result) and x^-4.3 = q^4 t, x^-1.7 = q^2 / t
+        // in FP64 (a handful of FP64 operations once per photon)
+        const double q = 550.0 / (double)wl;
+        const double q2 = q * q;
+        const double t = (double)exp2f(0.3f * log2f((float)q));
+        const double b = fma((double)m.kop1, q2 * q2 * t, fma((double)m.kop2, q2 / t,
+                                                                 (double)m.kop3 * t));
+        v.sca_len = (float)((double)m.sca_scale / b);
         const float il = 1.0f / wl;
-        n = m.qf0 + il * (m.qf1 + il * (m.qf2 + il * m.qf3));
-        const float ng = m.qf0 + il * (2.0f * m.qf1 + il * (3.0f * m.qf2 + il * (4.0f * m.qf3)));
+        // Horner with FMAs
+        n = fmaf(il, fmaf(il, fmaf(il, m.qf3, m.qf2), m.qf1), m.qf0);
+        const float ng =
+            fmaf(il, fmaf(il, fmaf(il, 4.0f * m.qf3, 3.0f * m.qf2), 2.0f * m.qf1), m.qf0);
         v.inv_c_grp = ng * (1.0f / kCVacMNs);
     }
     v.cos_c = 1.0f / n;  // cherenkov_angle: acos(1/n), beta = 1
@@ -353,14 +369,16 @@ __device__ __forceinline__ float normal_sample(const Stream& s, uint32_t sub) {
 }
 
 // Marsaglia-Tsang gamma(shape, scale). Each sampler block: a Box-Muller pair
-// (angle 2 pi u, sincos_2pi) and two acceptance uniforms (two attempts).
+// (angle 2 pi u) and two acceptance uniforms (two attempts).
 // shape < 1 boosts Gamma(shape+1) by U^(1/shape), U from the sub-stream's
 // boost block. The accepted value d (1 + c x)^3 scale boost amplifies the
 // relative error of 1 + c x three times, so that factor is formed with one
 // FMA from the float variate and the cube and the product are taken in FP64
 // (a handful of FP64 operations once per photon) and rounded once: a draw
 // whose decisions agree with an exact evaluation is then within ~2 ulp of it.
-__device__ inline float gamma_sample(float shape, float scale, const Stream& s, uint32_t sub) {
+// x_acc (may be NULL) receives the accepted normal variate (0 if none is).
+__device__ inline float gamma_sample(float shape, float scale, const Stream& s, uint32_t sub,
+                                     float* x_acc = nullptr) {
     float boost = 1.0f;
     if (shape < 1.0f) {
         const uint4 r = s.block(sampler_word(sub, PP_SAMPLER_BOOST_BLOCK));
@@ -372,9 +390,12 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
     const float c = 1.0f / sqrtf(9.0f * d);
     for (uint32_t i = 0; i < PP_GAMMA_MAX_BLOCKS; ++i) {
         const uint4 r = s.block(sampler_word(sub, i));
-        const float rad = sqrt_hw(-2.0f * logf(u01_open0(r.x)));
+        // the Box-Muller radius and angle from the library logarithm, a
+        // correctly rounded square root and sincospi (each within ~1 ulp):
+        // the cube below amplifies the variate's error
+        const float rad = sqrtf(-2.0f * logf(u01_open0(r.x)));
         float sp, cp;
-        sincos_2pi(u01(r.y), &sp, &cp);
+        sincospif(2.0f * u01(r.y), &sp, &cp);
         const float z[2] = {rad * cp, rad * sp};
         const float ua[2] = {u01_open0(r.z), u01_open0(r.w)};
 #pragma unroll
@@ -386,10 +407,13 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
             const float v = (float)v3;
             const float u = ua[k];
             const float x2 = x * x;
-            if (u < 1.0f - 0.0331f * (x2 * x2) || logf(u) < 0.5f * x2 + d * (1.0f - v + logf(v)))
+            if (u < 1.0f - 0.0331f * (x2 * x2) || logf(u) < 0.5f * x2 + d * (1.0f - v + logf(v))) {
+                if (x_acc) *x_acc = x;
                 return (float)(dd * v3 * (double)scale * (double)boost);
+            }
         }
     }
+    if (x_acc) *x_acc = 0.0f;
     return (float)(dd * (double)scale * (double)boost);
 }
 
@@ -399,15 +423,14 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
 __device__ __forceinline__ float spectrum_sample(const float* tab, int n, int mapping, float mono,
                                                  float u) {
     if (n == 0) return mono;
-    // separately rounded products and sums (no contraction): the knot
-    // position and a + f (b - a) as the reference evaluates them
-    float x = mapping == PP_SPEC_TEXTURE ? __fadd_rn(__fmul_rn(u, (float)n), -0.5f)
-                                         : __fmul_rn(u, (float)(n - 1));
+    // separately rounded products and sums (this header turns contraction
+    // off): the knot position and a + f (b - a) as the reference evaluates them
+    float x = mapping == PP_SPEC_TEXTURE ? u * (float)n - 0.5f : u * (float)(n - 1);
     x = fminf(fmaxf(x, 0.0f), (float)(n - 1));
     const int i = min((int)x, n - 2);
     const float f = x - (float)i;
     const float a = tab[i], b = tab[i + 1];
-    return __fadd_rn(a, __fmul_rn(f, b - a));
+    return a + f * (b - a);
 }
 
 }  // namespace pp

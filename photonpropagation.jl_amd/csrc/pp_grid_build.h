@@ -150,6 +150,9 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         float r_max = 0.0f;
         for (int i = 0; i < n; ++i) r_max = std::max(r_max, radius[i]);
         g.zpad = r_max + mb;
+        g.ex = g.ox + (float)g.nx * g.h;
+        g.ey = g.oy + (float)g.ny * g.h;
+        g.ez = g.oz + (float)g.nz * g.hz;
         return true;
     };
     const float hz_req = cell_z_req > 0.0f ? std::max(cell_z_req, 1e-3f) : 0.0f;

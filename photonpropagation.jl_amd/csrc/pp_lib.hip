@@ -22,9 +22,10 @@
 
 #include "pp_device.h"
 #include "pp_grid_build.h"
+#include "pp_hits.h"
 #include "pp_search.h"
 
-#include <hipcub/device/device_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 using pp::f3;
 
@@ -33,21 +34,9 @@ using pp::f3;
 // ---------------------------------------------------------------------------
 namespace {
 
-// One hit record: 64 bytes, 64-byte aligned, so that a hit is written as two
-// full 32-byte sectors (four 16-byte stores by one lane).
-//   f[0..2] position, f[3..5] direction, f[6..8] initial direction,
-//   f[9] wavelength, f[10] time, f[11] distance travelled,
-//   id = photon id (bits 0..39) | module index, 1-based (bits 40..55),
-//   n_steps, source index.
-struct alignas(64) HitRecord {
-    float f[12];
-    uint64_t id;
-    int32_t n_steps;
-    int32_t source_ix;
-};
-static_assert(sizeof(HitRecord) == 64, "hit record is 64 bytes");
-constexpr uint64_t kPidBits = 40;
-constexpr uint64_t kMaxPhotonsPerSource = 1ull << kPidBits;
+using pp::HitRecord;
+using pp::kMaxPhotonsPerSource;
+using pp::kPidBits;
 constexpr int kMaxTargets = 65535;
 
 struct HitCols {
@@ -83,6 +72,7 @@ struct KParams {
     const float* spec;
     HitCols hits;
     Counters* counters;
+    uint32_t queue_off;  // LDS byte offset of the waves' walk queues (grid searches)
 };
 
 // ---------------------------------------------------------------------------
@@ -94,8 +84,9 @@ struct PhotonInit {
     pp::MediumValues mv;
 };
 
+// p + s d, each component one FMA (the same bits wherever it is inlined)
 __device__ __forceinline__ f3 add_scaled(f3 p, float s, f3 d) {
-    return pp::make3(p.x + s * d.x, p.y + s * d.y, p.z + s * d.z);
+    return pp::make3(fmaf(s, d.x, p.x), fmaf(s, d.y, p.y), fmaf(s, d.z, p.z));
 }
 
 // sample_cherenkov_direction[_no_smear|_custom_smear]  :80-111, :134-154
@@ -231,6 +222,10 @@ using pp::lds_bytes;
 // and the vector cache) instead of LDS copies, for target lists whose records
 // and grid tables exceed the LDS
 enum { kSearchBrute = 0, kSearchGrid = 1, kSearchMixed = 2, kSearchGridGlobal = 3 };
+// the grid searches walk in batches (transport_group); the every-target
+// scans test each step at once (transport_photon)
+template <int SEARCH>
+constexpr bool kDeferredWalk = SEARCH == kSearchGrid || SEARCH == kSearchGridGlobal;
 
 // LDS tables of one workgroup: targets, grid cells and items, spectrum knots
 struct Lds {
@@ -290,6 +285,14 @@ __device__ __forceinline__ SearchT<SEARCH> make_search(const KParams& p, const L
     }
 }
 
+// position, distance and time after moving d along dir
+__device__ __forceinline__ void move_by(f3& pos, float& dist, float& time, f3 dir, float d,
+                                        float inv_c) {
+    pos = add_scaled(pos, d, dir);
+    dist += d;
+    time = fmaf(d, inv_c, time);
+}
+
 // One photon's life: emission, then up to n_steps steps of
 // src/photon_prop_cuda.jl:486-539. Returns true on a hit; h holds the record.
 template <bool COUNT, typename Search>
@@ -313,21 +316,170 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
         const int found = search.template find<COUNT>(pos, dir, step_size, dt, work);
         if (COUNT) ++n_steps_done;
         if (found >= 0) {
-            pos = add_scaled(pos, dt, dir);
-            dist += dt;
-            time += dt * inv_c;
+            move_by(pos, dist, time, dir, dt, inv_c);
             hit = true;
             module_ix = found + 1;
             n_at = step;
             break;
         }
-        pos = add_scaled(pos, step_size, dir);
-        dist += step_size;
-        time += step_size * inv_c;
+        move_by(pos, dist, time, dir, step_size, inv_c);
         const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
         dir = pp::update_direction(dir, ct, pp::u01(r.w));
     }
     h = HitRec{pos, dir, st.dir, st.wl, time, dist, module_ix, n_at, pid};
+    return hit;
+}
+
+// ---------------------------------------------------------------------------
+// Deferred walk of the grid searches. A wave transports its 64 photons step
+// by step as in transport_photon, but a step whose segment meets the grid box
+// does not walk the grid at once: the segment becomes a job in the wave's LDS
+// queue, the photon moves on as if nothing were hit, and whenever 64 jobs are
+// queued every lane of the wave walks one of them. The walks of a step run
+// with 20-30% of the lanes in the old schedule (the others are outside the
+// grid or done); in a batch every lane walks. A job that hits records
+// (step << 16 | target) in its photon's LDS slot with an atomic minimum; the
+// earliest step wins, which is the step at which transport_photon stops (its
+// later steps were never taken by the reference and their jobs only lose the
+// minimum). At the end the few photons with a hit replay their steps up to
+// it (same random stream, same arithmetic) for the hit record. The result is
+// that of transport_photon bit for bit.
+// ---------------------------------------------------------------------------
+// Jobs per wave: a batch of up to 64 runs at the start of a step whenever 64
+// or more are pending or the step's new jobs would not fit, so at most kQueue
+// are pending after any step.
+constexpr uint32_t kQueue = 96;
+constexpr uint32_t kWaveQueueBytes = kQueue * 32 + 64 * 8 + 64 * 4;
+constexpr int32_t kDeferredMaxSteps = (1 << 26) - 1;  // step index of a job: 26 bits
+
+struct WaveQueue {
+    float4* job;               // 2 per job: (position, step length), (direction, lane | step << 6)
+    unsigned long long* key;   // per lane: earliest (step << 16 | target) hit, ~0 if none
+    float* dt;                 // per lane: distance to that hit within its step
+};
+
+__device__ __forceinline__ WaveQueue wave_queue(const KParams& p) {
+    extern __shared__ float4 lds[];
+    char* base = reinterpret_cast<char*>(lds) + p.queue_off + (threadIdx.x >> 6) * kWaveQueueBytes;
+    WaveQueue q;
+    q.job = reinterpret_cast<float4*>(base);
+    q.key = reinterpret_cast<unsigned long long*>(base + kQueue * 32);
+    q.dt = reinterpret_cast<float*>(base + kQueue * 32 + 64 * 8);
+    return q;
+}
+
+// Orders the wave's LDS accesses: the LDS executes one wave's accesses in
+// issue order, so only the compiler must not move them across this point.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One photon per lane (active lanes), the whole wave together.
+template <bool COUNT, typename Search>
+__device__ __forceinline__ bool transport_group(const KParams& p, const pp_source& src,
+                                                uint32_t src_ix, uint64_t pid, bool active,
+                                                const float* spec, const Search& search,
+                                                const pp::Phase& phase, const WaveQueue& q,
+                                                pp::Work& work, unsigned long long& n_steps_done,
+                                                HitRec& h) {
+    const uint32_t lane = __lane_id();
+    pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), src_ix};
+    PhotonInit st{};
+    if (active) st = init_photon(p, src, spec, s);
+    q.key[lane] = ~0ull;
+    f3 pos = st.pos, dir = st.dir;
+    const float ln2_sca = st.mv.sca_len * pp::kLn2, inv_c = st.mv.inv_c_grp;
+    // going: still stepping (active, and no hit found so far; steps after a
+    // found hit cannot change the earliest one)
+    bool going = active, stop = false;
+    uint32_t head = 0, count = 0;  // wave-uniform ring of pending jobs
+    // every lane walks one pending job (lanes >= n none)
+    auto walk_jobs = [&](uint32_t n) {
+        wave_sync();
+        int found = -1;
+        float dt = 0.0f;
+        unsigned long long key = 0;
+        uint32_t owner = 0;
+        if (lane < n) {
+            uint32_t slot = head + lane;
+            if (slot >= kQueue) slot -= kQueue;
+            const float4 a = q.job[2 * slot], b = q.job[2 * slot + 1];
+            const uint32_t meta = __float_as_uint(b.w);
+            owner = meta & 63u;
+            found = search.template find<COUNT>(pp::make3(a.x, a.y, a.z), pp::make3(b.x, b.y, b.z),
+                                                a.w, dt, work);
+            if (found >= 0) {
+                key = ((unsigned long long)(meta >> 6) << 16) | (uint32_t)found;
+                atomicMin(q.key + owner, key);
+            }
+        }
+        wave_sync();
+        if (found >= 0 && q.key[owner] == key) q.dt[owner] = dt;
+        wave_sync();
+        stop = q.key[lane] != ~0ull;
+        head += n;
+        if (head >= kQueue) head -= kQueue;
+        count -= n;
+    };
+    // one loop for the steps and the final batches, so that the walk is
+    // inlined once
+    int32_t step = 0;
+    bool stepping = true;  // wave-uniform
+    for (;;) {
+        going = going && !stop;
+        stepping = stepping && step < p.n_steps && __ballot(going) != 0;
+        if (!stepping && count == 0) break;
+        const bool move = stepping && going;
+        if (stepping) ++step;
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        float step_size = 0.0f;
+        bool need = false;
+        if (move) {
+            r = s.block((uint32_t)step);
+            step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
+            const pp::Clip c = search.clip(pos, dir, step_size);
+            need = c.t_in <= c.t_out;
+        }
+        const uint64_t m = __ballot(need);
+        const uint32_t n_new = (uint32_t)__popcll(m);
+        if (!stepping || count >= 64 || count + n_new > kQueue) walk_jobs(count < 64 ? count : 64);
+        if (need) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            uint32_t slot = head + count + rank;
+            if (slot >= kQueue) slot -= kQueue;
+            q.job[2 * slot] = make_float4(pos.x, pos.y, pos.z, step_size);
+            q.job[2 * slot + 1] =
+                make_float4(dir.x, dir.y, dir.z, __uint_as_float(lane | ((uint32_t)step << 6)));
+        }
+        count += n_new;
+        if (move) {
+            pos = add_scaled(pos, step_size, dir);
+            const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
+            dir = pp::update_direction(dir, ct, pp::u01(r.w));
+        }
+    }
+    wave_sync();
+    const unsigned long long key = q.key[lane];
+    const bool hit = active && key != ~0ull;
+    if (COUNT && active) n_steps_done += hit ? (unsigned long long)(key >> 16) : (unsigned long long)p.n_steps;
+    if (hit) {
+        // the photon's steps again, up to the one that hits
+        const int32_t k = (int32_t)(key >> 16);
+        const float dtk = q.dt[lane];
+        f3 rp = st.pos, rd = st.dir;
+        float rt = st.time, rdist = 0.0f;
+        for (int32_t i = 1; i < k; ++i) {
+            const uint4 r = s.block((uint32_t)i);
+            const float step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
+            move_by(rp, rdist, rt, rd, step_size, inv_c);
+            const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
+            rd = pp::update_direction(rd, ct, pp::u01(r.w));
+        }
+        move_by(rp, rdist, rt, rd, dtk, inv_c);
+        h = HitRec{rp, rd, st.dir, st.wl, rt, rdist, (int32_t)(key & 0xFFFFu) + 1, k, pid};
+    }
     return hit;
 }
 
@@ -363,16 +515,37 @@ __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long n_done = 0, n_steps_done = 0;
     pp::Work work;
-    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < p.n_photons;
-         pid += stride) {
-        HitRec h;
-        const bool hit = transport_photon<COUNT>(p, p.src, p.src_ix, pid, l.spec, search, phase,
-                                                 work, n_steps_done, h);
-        if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
-            atomicOr(&p.counters->err, 1u);
-            break;
+    if constexpr (kDeferredWalk<SEARCH>) {
+        // whole waves of photons, the loop condition uniform across the wave
+        const WaveQueue q = wave_queue(p);
+        bool overflow = false;
+        for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+             base < p.n_photons; base += stride) {
+            const uint64_t pid = base + __lane_id();
+            const bool active = pid < p.n_photons && !overflow;
+            HitRec h;
+            const bool hit = transport_group<COUNT>(p, p.src, p.src_ix, pid, active, l.spec, search,
+                                                    phase, q, work, n_steps_done, h);
+            if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
+                atomicOr(&p.counters->err, 1u);
+                overflow = true;
+            } else if (active) {
+                ++n_done;
+            }
+            if (__ballot(overflow)) break;
         }
-        ++n_done;
+    } else {
+        for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < p.n_photons;
+             pid += stride) {
+            HitRec h;
+            const bool hit = transport_photon<COUNT>(p, p.src, p.src_ix, pid, l.spec, search,
+                                                     phase, work, n_steps_done, h);
+            if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
+                atomicOr(&p.counters->err, 1u);
+                break;
+            }
+            ++n_done;
+        }
     }
     flush_counts<COUNT>(p, n_done, n_steps_done, work);
 }
@@ -400,6 +573,8 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
     const auto search = make_search<SEARCH>(p, l);
     const uint64_t total = b.chunk_prefix[b.n_sources];
     const uint32_t lane = __lane_id();
+    WaveQueue q{};
+    if constexpr (kDeferredWalk<SEARCH>) q = wave_queue(p);
     unsigned long long n_done = 0, n_steps_done = 0;
     pp::Work work;
     bool overflow = false;
@@ -423,16 +598,29 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
         const uint64_t end = first + b.chunk < n_src ? first + b.chunk : n_src;
         for (uint64_t base = first; base < end; base += 64) {
             const uint64_t pid = base + lane;
-            if (pid >= end || overflow) continue;
-            HitRec h;
-            const bool hit = transport_photon<COUNT>(p, src, (uint32_t)si, pid, l.spec, search,
-                                                     phase, work, n_steps_done, h);
-            if (!store_hits(hit, h, (uint32_t)si, p.hits, p.counters)) {
-                atomicOr(&p.counters->err, 1u);
-                overflow = true;
-                continue;
+            if constexpr (kDeferredWalk<SEARCH>) {
+                const bool active = pid < end && !overflow;
+                HitRec h;
+                const bool hit = transport_group<COUNT>(p, src, (uint32_t)si, pid, active, l.spec,
+                                                        search, phase, q, work, n_steps_done, h);
+                if (!store_hits(hit, h, (uint32_t)si, p.hits, p.counters)) {
+                    atomicOr(&p.counters->err, 1u);
+                    overflow = true;
+                } else if (active) {
+                    ++n_done;
+                }
+            } else {
+                if (pid >= end || overflow) continue;
+                HitRec h;
+                const bool hit = transport_photon<COUNT>(p, src, (uint32_t)si, pid, l.spec, search,
+                                                         phase, work, n_steps_done, h);
+                if (!store_hits(hit, h, (uint32_t)si, p.hits, p.counters)) {
+                    atomicOr(&p.counters->err, 1u);
+                    overflow = true;
+                    continue;
+                }
+                ++n_done;
             }
-            ++n_done;
         }
         if (__ballot(overflow)) break;  // the call fails; stop taking chunks
     }
@@ -526,7 +714,9 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
     }
     case PP_FN_GAMMA: {
         pp::Stream s{bits(x[2]), bits(x[3]), bits(x[4]), bits(x[5]), bits(x[6])};
-        y[0] = pp::gamma_sample(x[0], x[1], s, PP_SUB_LONG_GAMMA);
+        float xa = 0.0f;
+        y[0] = pp::gamma_sample(x[0], x[1], s, PP_SUB_LONG_GAMMA, &xa);
+        y[1] = xa;
         break;
     }
     case PP_FN_SPECTRUM:
@@ -659,6 +849,11 @@ struct pp_ctx {
     void* d_batch = nullptr;
     void* h_batch = nullptr;
     size_t batch_cap = 0;  // bytes of each
+    // PMT detection of the context's hits: targets as (x, y, z, radius), and
+    // one device block holding the acceptance table | detected count | PMT ids
+    float4* d_tgt_r = nullptr;
+    char* d_pmt = nullptr;
+    size_t pmt_cap = 0;
     // hipGraph of the batch mode PP_BATCH_GRAPH: counter clear -> propagation
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -679,7 +874,11 @@ int set_dev(pp_ctx* c) {
 }
 
 constexpr int kMaxLdsBytes = 160 * 1024;
+// records per pinned staging chunk of pp_copy_hits (64 MB)
+constexpr int64_t kStageRecords = (int64_t)1 << 20;
 constexpr size_t kGridLdsBudget = 64 * 1024;  // grid tables + targets + spectrum per block
+// the largest table set that leaves room for the walk queues of 16 waves
+constexpr size_t kGridLdsMax = kMaxLdsBytes - 16 * (size_t)kWaveQueueBytes;
 constexpr size_t kGridGlobalBudget = (size_t)64 << 20;  // tables read from device memory
 
 // (Re)build the target grid for the requested cell edge and upload it.
@@ -698,7 +897,7 @@ int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
     if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
                                c->spec_n, kGridLdsBudget, t) &&
         !pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req, cell_z_req,
-                               c->spec_n, kMaxLdsBytes, t)) {
+                               c->spec_n, kGridLdsMax, t)) {
         global = true;
         if (!pp::build_grid_tables(xyz.data(), c->h_radius.data(), c->n_targets, cell_req,
                                    cell_z_req, c->spec_n, kGridGlobalBudget, t))
@@ -1031,6 +1230,8 @@ int pp_destroy(pp_ctx* ctx) {
     if (ctx->h_batch) (void)hipHostFree(ctx->h_batch);
     if (ctx->h_rec) (void)hipHostFree(ctx->h_rec);
     if (ctx->d_sort) (void)hipFree(ctx->d_sort);
+    if (ctx->d_tgt_r) (void)hipFree(ctx->d_tgt_r);
+    if (ctx->d_pmt) (void)hipFree(ctx->d_pmt);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1143,15 +1344,26 @@ int pp_set_targets(pp_ctx* ctx, const pp_target* t, int n) {
                                          : make_float4(kf, 0.0f, 0.0f, 0.0f);
         }
     }
+    // (x, y, z, radius) per target for the PMT detection of hits
+    std::vector<float4> hr((size_t)std::max(n, 1));
+    for (int i = 0; i < n; ++i)
+        hr[(size_t)i] = make_float4(t[i].position[0], t[i].position[1], t[i].position[2],
+                                    t[i].radius);
     if (n * per > ctx->tgt_cap) {
         if (ctx->d_targets) PP_HIP(hipFree(ctx->d_targets));
-        ctx->d_targets = nullptr;
+        if (ctx->d_tgt_r) PP_HIP(hipFree(ctx->d_tgt_r));
+        ctx->d_targets = ctx->d_tgt_r = nullptr;
+        ctx->tgt_cap = 0;
         PP_HIP(hipMalloc(&ctx->d_targets, (size_t)n * per * sizeof(float4)));
+        PP_HIP(hipMalloc(&ctx->d_tgt_r, (size_t)n * per * sizeof(float4)));
         ctx->tgt_cap = n * per;
     }
-    if (n > 0)
+    if (n > 0) {
         PP_HIP(hipMemcpyAsync(ctx->d_targets, h.data(), (size_t)n * per * sizeof(float4),
                               hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(ctx->d_tgt_r, hr.data(), (size_t)n * sizeof(float4),
+                              hipMemcpyHostToDevice, ctx->stream));
+    }
     PP_HIP(hipStreamSynchronize(ctx->stream));
     ctx->n_targets = n;
     ctx->planar = planar;
@@ -1209,6 +1421,11 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     if (automatic)
         search = ctx->n_targets > 8 && !ctx->planar ? PP_SEARCH_GRID : PP_SEARCH_BRUTE;
     if (ctx->n_targets == 0) search = PP_SEARCH_BRUTE;
+    if (search == PP_SEARCH_GRID && n_steps > kDeferredMaxSteps) {
+        if (!automatic)
+            return fail(PP_ERR_INVALID, "pp_propagate: the grid search takes at most 2^26 - 1 steps");
+        search = PP_SEARCH_BRUTE;
+    }
     if (search == PP_SEARCH_GRID) {
         const float want = ctx->cfg.grid_cell, want_z = ctx->cfg.grid_cell_z;
         if (ctx->grid_cell_built != want || ctx->grid_cell_z_built != want_z ||
@@ -1224,12 +1441,16 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     const int skind = global ? kSearchGridGlobal
                       : grid ? kSearchGrid : ctx->planar ? kSearchMixed : kSearchBrute;
     // the global grid stages only the spectrum
-    const size_t lds = global ? std::max<size_t>((size_t)ctx->spec_n * 4, 16)
-                              : lds_bytes(ctx->planar ? 2 * ctx->n_targets : ctx->n_targets,
-                                          grid ? ctx->grid.n_cells : 0,
-                                          grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
+    size_t lds = global ? std::max<size_t>((size_t)ctx->spec_n * 4, 16)
+                        : lds_bytes(ctx->planar ? 2 * ctx->n_targets : ctx->n_targets,
+                                    grid ? ctx->grid.n_cells : 0,
+                                    grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
+    // the grid kernels' per-wave walk queues follow the tables
+    const uint32_t queue_off = (uint32_t)((lds + 15) & ~(size_t)15);
+    if (grid) lds = queue_off + (size_t)(ctx->cfg.block_size / 64) * kWaveQueueBytes;
     if (lds > (size_t)kMaxLdsBytes)
-        return fail(PP_ERR_INVALID, "pp_propagate: targets and tables exceed the LDS budget");
+        return fail(PP_ERR_INVALID, "pp_propagate: targets, tables and walk queues exceed the LDS "
+                                    "of a workgroup (a smaller block_size may fit)");
     const int phase = ctx->medium.phase.kind;
     const bool count = ctx->cfg.count_work != 0;
     ctx->last_search = search;
@@ -1238,6 +1459,7 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     kp.k0 = (uint32_t)seed;
     kp.k1 = (uint32_t)(seed >> 32);
     kp.n_steps = n_steps;
+    kp.queue_off = queue_off;
     kp.targets = ctx->d_targets;
     kp.n_targets = ctx->n_targets;
     if (grid) {
@@ -1464,9 +1686,9 @@ static int sort_hits_device(pp_ctx* ctx, bool* done) {
     int end_bit = (int)kPidBits;
     for (uint32_t v = (uint32_t)(ns - 1); v; v >>= 1) ++end_bit;
     size_t temp = 0;
-    PP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint64_t*)nullptr,
-                                              (uint64_t*)nullptr, (const uint32_t*)nullptr,
-                                              (uint32_t*)nullptr, (int)n, 0, end_bit, ctx->stream));
+    PP_HIP(rocprim::radix_sort_pairs(nullptr, temp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)n, 0u,
+                                     (unsigned)end_bit, ctx->stream));
     const size_t br = align256((size_t)n * sizeof(HitRecord));
     const size_t bk = align256((size_t)n * 8), bv = align256((size_t)n * 4);
     const size_t need = br + 2 * bk + 2 * bv + align256(temp);
@@ -1489,8 +1711,8 @@ static int sort_hits_device(pp_ctx* ctx, bool* done) {
     void* tmp = base + 2 * bk + 2 * bv;
     hit_keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(ctx->hits.rec, n, k_in, v_in);
     PP_HIP(hipGetLastError());
-    PP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, v_out, (int)n, 0,
-                                              end_bit, ctx->stream));
+    PP_HIP(rocprim::radix_sort_pairs(tmp, temp, k_in, k_out, v_in, v_out, (uint32_t)n, 0u,
+                                     (unsigned)end_bit, ctx->stream));
     gather_hits_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
         ctx->hits.rec, v_out, n, (HitRecord*)ctx->d_sort);
     PP_HIP(hipGetLastError());
@@ -1520,39 +1742,20 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
     // records to read: [lo, lo+cnt) of the device buffer src
     const HitRecord* src = ordered && !host_sort ? (const HitRecord*)ctx->d_sort : ctx->hits.rec;
     const int64_t lo = host_sort ? 0 : first, cnt = host_sort ? ctx->n_hits : n;
-    // records through a pinned staging buffer (kept by the context)
-    if (cnt > ctx->h_rec_cap) {
+    // records stream through a pinned staging buffer of at most
+    // kStageRecords records (kept by the context); the host sort, which
+    // needs random access to every record, collects them in pageable memory
+    const int64_t stage = std::min<int64_t>(cnt, kStageRecords);
+    if (stage > ctx->h_rec_cap) {
         if (ctx->h_rec) PP_HIP(hipHostFree(ctx->h_rec));
         ctx->h_rec = nullptr;
         ctx->h_rec_cap = 0;
-        PP_HIP(hipHostMalloc((void**)&ctx->h_rec, (size_t)cnt * sizeof(HitRecord), hipHostMallocDefault));
-        ctx->h_rec_cap = cnt;
+        PP_HIP(hipHostMalloc((void**)&ctx->h_rec, (size_t)stage * sizeof(HitRecord),
+                             hipHostMallocDefault));
+        ctx->h_rec_cap = stage;
     }
-    const HitRecord* rec = ctx->h_rec;
-    PP_HIP(hipMemcpyAsync(ctx->h_rec, src + lo, (size_t)cnt * sizeof(HitRecord),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    PP_HIP(hipStreamSynchronize(ctx->stream));
     const uint64_t pid_mask = kMaxPhotonsPerSource - 1;
-    if (host_sort && !ctx->perm_valid) {
-        // sort (source, photon id) keys beside their record index: one
-        // contiguous array instead of indirect two-field compares
-        struct Key {
-            int32_t src;
-            uint64_t pid;
-            int64_t idx;
-        };
-        std::vector<Key> keys((size_t)ctx->n_hits);
-        for (int64_t k = 0; k < ctx->n_hits; ++k)
-            keys[(size_t)k] = {rec[k].source_ix, rec[k].id & pid_mask, k};
-        std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
-            return a.src != b.src ? a.src < b.src : a.pid < b.pid;
-        });
-        ctx->perm.resize((size_t)ctx->n_hits);
-        for (int64_t k = 0; k < ctx->n_hits; ++k) ctx->perm[(size_t)k] = keys[(size_t)k].idx;
-        ctx->perm_valid = true;
-    }
-    for (int64_t k = 0; k < n; ++k) {
-        const HitRecord& r = rec[(size_t)(host_sort ? ctx->perm[(size_t)(first + k)] : k)];
+    auto scatter = [&](const HitRecord& r, int64_t k) {
         for (int c = 0; c < 3; ++c) {
             if (pos3) pos3[3 * k + c] = r.f[c];
             if (dir3) dir3[3 * k + c] = r.f[3 + c];
@@ -1570,6 +1773,165 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
         if (n_steps) n_steps[k] = r.n_steps;
         if (photon_id) photon_id[k] = r.id & pid_mask;
         if (source_ix) source_ix[k] = r.source_ix;
+    };
+    std::vector<HitRecord> all;
+    if (host_sort) all.resize((size_t)cnt);
+    for (int64_t off = 0; off < cnt; off += stage) {
+        const int64_t m = std::min<int64_t>(stage, cnt - off);
+        PP_HIP(hipMemcpyAsync(ctx->h_rec, src + lo + off, (size_t)m * sizeof(HitRecord),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        if (host_sort)
+            std::memcpy(all.data() + off, ctx->h_rec, (size_t)m * sizeof(HitRecord));
+        else
+            for (int64_t k = 0; k < m; ++k) scatter(ctx->h_rec[k], off + k);
+    }
+    if (!host_sort) return PP_OK;
+    if (!ctx->perm_valid) {
+        // sort (source, photon id) keys beside their record index: one
+        // contiguous array instead of indirect two-field compares
+        struct Key {
+            int32_t src;
+            uint64_t pid;
+            int64_t idx;
+        };
+        std::vector<Key> keys((size_t)ctx->n_hits);
+        for (int64_t k = 0; k < ctx->n_hits; ++k)
+            keys[(size_t)k] = {all[(size_t)k].source_ix, all[(size_t)k].id & pid_mask, k};
+        std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+            return a.src != b.src ? a.src < b.src : a.pid < b.pid;
+        });
+        ctx->perm.resize((size_t)ctx->n_hits);
+        for (int64_t k = 0; k < ctx->n_hits; ++k) ctx->perm[(size_t)k] = keys[(size_t)k].idx;
+        ctx->perm_valid = true;
+    }
+    for (int64_t k = 0; k < n; ++k) scatter(all[(size_t)ctx->perm[(size_t)(first + k)]], k);
+    return PP_OK;
+}
+
+}  // extern "C"
+
+int pp::ctx_hit_records(pp_ctx* ctx, const HitRecord** rec, int64_t* n, hipStream_t* stream,
+                        int* device) {
+    if (!ctx) return fail(PP_ERR_INVALID, "NULL ctx");
+    if (int rc = set_dev(ctx)) return rc;
+    *rec = ctx->hits.rec;
+    *n = ctx->n_hits;
+    *stream = ctx->stream;
+    *device = ctx->device;
+    return PP_OK;
+}
+
+extern "C" {
+
+int pp_release_scratch(pp_ctx* ctx) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_release_scratch: NULL ctx");
+    if (int rc = set_dev(ctx)) return rc;
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_sort) PP_HIP(hipFree(ctx->d_sort));
+    if (ctx->h_rec) PP_HIP(hipHostFree(ctx->h_rec));
+    if (ctx->d_pmt) PP_HIP(hipFree(ctx->d_pmt));
+    ctx->d_sort = nullptr;
+    ctx->d_sort_cap = 0;
+    ctx->sorted_valid = false;
+    ctx->h_rec = nullptr;
+    ctx->h_rec_cap = 0;
+    ctx->d_pmt = nullptr;
+    ctx->pmt_cap = 0;
+    std::vector<int64_t>().swap(ctx->perm);
+    ctx->perm_valid = false;
+    return PP_OK;
+}
+
+int pp_pom_pmt_hits_ctx(pp_ctx* ctx, const pp_pom_model* m, const float* wl_grid,
+                        const float* acc1, const float* acc2, uint64_t seed, int64_t first,
+                        int64_t n, int32_t* pmt_id_out, int64_t* n_detected, float* kernel_ms) {
+    if (!ctx) return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_ctx: NULL ctx");
+    if (n_detected) *n_detected = 0;
+    if (kernel_ms) *kernel_ms = 0.0f;
+    if (!m || m->n_wl < 2 || m->n_wl > 1024)
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_ctx: model with n_wl in [2, 1024] required");
+    if (first < 0 || n < 0 || first > ctx->n_hits || n > ctx->n_hits - first)
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_ctx: range outside [0, n_hits)");
+    if (ctx->planar)
+        return fail(PP_ERR_INVALID,
+                    "pp_pom_pmt_hits_ctx: the target list holds planar targets (POMs are spheres)");
+    pp::PmtSetup st{};
+    std::vector<float> tab(3 * (size_t)m->n_wl);
+    if (int rc = pp::pmt_setup(m, wl_grid, acc1, acc2, seed, &st, tab.data())) return rc;
+    if (n == 0) return PP_OK;
+    if (!pmt_id_out) return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_ctx: NULL output");
+    if (int rc = set_dev(ctx)) return rc;
+    // the records in the order pp_copy_hits returns them
+    const bool ordered = ctx->cfg.deterministic_order != 0;
+    if (ordered && ctx->cfg.deterministic_order != 2 && !ctx->sorted_valid && !ctx->perm_valid) {
+        bool done = false;
+        if (int rc = sort_hits_device(ctx, &done)) return rc;
+        ctx->sorted_valid = done;
+    }
+    const bool host_sort = ordered && (ctx->cfg.deterministic_order == 2 || !ctx->sorted_valid);
+    const HitRecord* src = ordered && !host_sort ? (const HitRecord*)ctx->d_sort : ctx->hits.rec;
+    const int64_t lo = host_sort ? 0 : first, cnt = host_sort ? ctx->n_hits : n;
+    // table | detected count | PMT ids, kept by the context and grown when needed
+    const size_t off_cnt = align256(tab.size() * sizeof(float));
+    const size_t off_out = off_cnt + 256;
+    const size_t need = off_out + (size_t)cnt * sizeof(int32_t);
+    if (need > ctx->pmt_cap) {
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_pmt) PP_HIP(hipFree(ctx->d_pmt));
+        ctx->d_pmt = nullptr;
+        ctx->pmt_cap = 0;
+        if (hipMalloc((void**)&ctx->d_pmt, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(PP_ERR_NOMEM, "pp_pom_pmt_hits_ctx: device allocation failed");
+        }
+        ctx->pmt_cap = need;
+    }
+    float* d_tab = (float*)ctx->d_pmt;
+    auto* d_cnt = (unsigned long long*)(ctx->d_pmt + off_cnt);
+    auto* d_out = (int32_t*)(ctx->d_pmt + off_out);
+    PP_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    PP_HIP(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), ctx->stream));
+    PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (int rc = pp::pmt_launch_records(st, d_tab, src + lo, cnt, ctx->d_tgt_r, d_out, d_cnt,
+                                        ctx->stream, ctx->n_cu))
+        return rc;
+    PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<int32_t> all;
+    int32_t* dst = pmt_id_out;
+    if (host_sort) {
+        all.resize((size_t)cnt);
+        dst = all.data();
+    }
+    PP_HIP(hipMemcpyAsync(dst, d_out, (size_t)cnt * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    if (kernel_ms) PP_HIP(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1));
+    if (host_sort) {
+        if (!ctx->perm_valid) {
+            // the same (source, photon id) order as pp_copy_hits's host sort
+            std::vector<HitRecord> rec((size_t)ctx->n_hits);
+            PP_HIP(hipMemcpyAsync(rec.data(), ctx->hits.rec, rec.size() * sizeof(HitRecord),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<int64_t> perm((size_t)ctx->n_hits);
+            std::iota(perm.begin(), perm.end(), 0);
+            const uint64_t mask = kMaxPhotonsPerSource - 1;
+            std::sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+                const HitRecord &x = rec[(size_t)a], &y = rec[(size_t)b];
+                return x.source_ix != y.source_ix ? x.source_ix < y.source_ix
+                                                  : (x.id & mask) < (y.id & mask);
+            });
+            ctx->perm.swap(perm);
+            ctx->perm_valid = true;
+        }
+        for (int64_t k = 0; k < n; ++k) pmt_id_out[k] = all[(size_t)ctx->perm[(size_t)(first + k)]];
+    }
+    if (n_detected) {
+        int64_t c = 0;
+        for (int64_t k = 0; k < n; ++k) c += pmt_id_out[k] != 0;
+        *n_detected = c;
     }
     return PP_OK;
 }
@@ -1577,7 +1939,7 @@ int pp_copy_hits(pp_ctx* ctx, int64_t first, int64_t n, float* pos3, float* dir3
 int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, int out_stride,
                int64_t n) {
     static const int kIn[PP_FN_COUNT] = {2, 6, 2, 2, 5, 11, 2, 1, 2, 6, 7, 1, 2, 13, 5};
-    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 1, 1, 1, 2, 1};
+    static const int kOut[PP_FN_COUNT] = {3, 3, 1, 1, 3, 2, 1, 3, 3, 4, 2, 1, 1, 2, 1};
     if (!ctx) return fail(PP_ERR_INVALID, "pp_eval_fn: NULL ctx");
     if (fn < 0 || fn >= PP_FN_COUNT) return fail(PP_ERR_INVALID, "pp_eval_fn: unknown function");
     if (n < 0 || (n > 0 && (!in || !out)))

@@ -23,6 +23,7 @@
 #include "photonprop_hip.h"
 #include "photonprop_surrogate.h"
 #include "pp_device.h"
+#include "pp_hits.h"
 
 // error text shared with the Monte Carlo half of the library (pp_last_error)
 void pp_set_error_message(const char* msg);
@@ -1380,6 +1381,134 @@ int pp_sur_eval_pairs(pp_mlp* M, const pp_cascade* events, int n_events, const f
     PP_HIP(hipStreamSynchronize(M->stream));
     if (kernel_ms) PP_HIP(hipEventElapsedTime(kernel_ms, M->ev0, M->ev1));
     return PP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Training labels from the hit records of a propagation whose sources are
+// the samples: per hit the absorption weight w = exp(-dist / abs_len(wl))
+// (abs_len linear in the table, clamped at its ends, times abs_scale, in
+// FP64 as the host computes it) and the residual bin of
+// x = t - t_geo(source) + PP_SUR_TIME_SHIFT. Weights are summed as 2^-32
+// fixed point in 64-bit integer atomics, so that the sums do not depend on
+// the order in which hits arrive. acc: (1 + PP_SUR_BINS) per source.
+constexpr int kLabelMaxKnots = 1024;
+constexpr double kFix = 4294967296.0;  // 2^32
+
+__global__ void __launch_bounds__(256) hit_label_kernel(const pp::HitRecord* __restrict__ rec,
+                                                        int64_t n, const double* tab, int n_abs,
+                                                        double abs_scale, const double* tgeo,
+                                                        int n_src, unsigned long long* acc) {
+    __shared__ double t[2 * kLabelMaxKnots];
+    for (int i = threadIdx.x; i < 2 * n_abs; i += blockDim.x) t[i] = tab[i];
+    __syncthreads();
+    const double* xw = t;
+    const double* yl = t + n_abs;
+    const double bin_w = (double)PP_SUR_TMAX / PP_SUR_BINS;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float4* r = reinterpret_cast<const float4*>(rec + i);
+        const float4 c = r[2], d = r[3];
+        const int s = (int)__float_as_uint(d.w);
+        if (s < 0 || s >= n_src) continue;
+        const double wl = c.y, dist = c.w, time = c.z;
+        // np.interp: clamped ends, slope * (x - x_j) + y_j inside
+        double len;
+        if (wl <= xw[0]) {
+            len = yl[0];
+        } else if (wl >= xw[n_abs - 1]) {
+            len = yl[n_abs - 1];
+        } else {
+            int lo = 0, hi = n_abs - 1;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (xw[mid] <= wl) lo = mid;
+                else hi = mid;
+            }
+            const double slope = (yl[hi] - yl[lo]) / (xw[hi] - xw[lo]);
+            len = slope * (wl - xw[lo]) + yl[lo];
+        }
+        const double w = exp(-dist / (len * abs_scale));
+        const unsigned long long wf = (unsigned long long)llrint(w * kFix);
+        unsigned long long* row = acc + (int64_t)s * (1 + PP_SUR_BINS);
+        atomicAdd(row, wf);
+        const double x = time - tgeo[s] + (double)PP_SUR_TIME_SHIFT;
+        const double b = floor(x / bin_w);
+        if (b >= 0.0 && b < (double)PP_SUR_BINS) atomicAdd(row + 1 + (int)b, wf);
+    }
+}
+
+__global__ void label_finish_kernel(const unsigned long long* acc, int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)((double)acc[i] / kFix);
+}
+
+int label_fail(int code, const std::string& msg) {
+    pp_set_error_message(msg.c_str());
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_sur_hit_labels(pp_ctx* ctx, const double* abs_wl, const double* abs_len, int n_abs,
+                      double abs_scale, const double* tgeo, int n_sources, float* labels) {
+    if (!ctx) return label_fail(PP_ERR_INVALID, "pp_sur_hit_labels: NULL ctx");
+    if (n_abs < 2 || n_abs > kLabelMaxKnots || !abs_wl || !abs_len)
+        return label_fail(PP_ERR_INVALID, "pp_sur_hit_labels: absorption table of 2..1024 knots");
+    for (int i = 1; i < n_abs; ++i)
+        if (!(abs_wl[i] > abs_wl[i - 1]))
+            return label_fail(PP_ERR_INVALID, "pp_sur_hit_labels: wavelengths must increase");
+    if (n_sources < 1 || !tgeo || !labels || !(abs_scale > 0.0))
+        return label_fail(PP_ERR_INVALID, "pp_sur_hit_labels: bad sources or output");
+    const pp::HitRecord* rec = nullptr;
+    int64_t n = 0;
+    hipStream_t st = nullptr;
+    int device = 0;
+    if (int rc = pp::ctx_hit_records(ctx, &rec, &n, &st, &device)) return rc;
+    const size_t per = 1 + PP_SUR_BINS;
+    const size_t acc_b = (size_t)n_sources * per * 8;
+    const size_t off_tab = (acc_b + 255) & ~(size_t)255;
+    const size_t off_tg = off_tab + (((size_t)2 * n_abs * 8 + 255) & ~(size_t)255);
+    const size_t off_out = off_tg + (((size_t)n_sources * 8 + 255) & ~(size_t)255);
+    const size_t bytes = off_out + (size_t)n_sources * per * 4;
+    char* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return label_fail(PP_ERR_NOMEM, "pp_sur_hit_labels: device allocation failed");
+    }
+    std::vector<double> tab(2 * (size_t)n_abs);
+    std::copy(abs_wl, abs_wl + n_abs, tab.begin());
+    std::copy(abs_len, abs_len + n_abs, tab.begin() + n_abs);
+    int rc = PP_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == PP_OK)
+            rc = label_fail(PP_ERR_HIP, std::string("pp_sur_hit_labels: ") + what + ": " +
+                                            hipGetErrorString(e));
+    };
+    chk(hipMemsetAsync(d, 0, acc_b, st), "clear");
+    chk(hipMemcpyAsync(d + off_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, st), "table");
+    chk(hipMemcpyAsync(d + off_tg, tgeo, (size_t)n_sources * 8, hipMemcpyHostToDevice, st), "tgeo");
+    if (rc == PP_OK && n > 0) {
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
+        hipLaunchKernelGGL(hit_label_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rec, n,
+                           (const double*)(d + off_tab), n_abs, abs_scale,
+                           (const double*)(d + off_tg), n_sources, (unsigned long long*)d);
+        chk(hipGetLastError(), "label kernel");
+    }
+    if (rc == PP_OK) {
+        const int64_t m = (int64_t)n_sources * per;
+        hipLaunchKernelGGL(label_finish_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           (const unsigned long long*)d, m, (float*)(d + off_out));
+        chk(hipGetLastError(), "label finish");
+        chk(hipMemcpyAsync(labels, d + off_out, (size_t)m * 4, hipMemcpyDeviceToHost, st), "copy");
+    }
+    chk(hipStreamSynchronize(st), "sync");
+    (void)hipFree(d);
+    return rc;
 }
 
 }  // extern "C"

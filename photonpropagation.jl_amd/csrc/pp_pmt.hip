@@ -11,6 +11,7 @@
 
 #include "photonprop_hip.h"
 #include "pp_device.h"
+#include "pp_hits.h"
 
 void pp_set_error_message(const char* msg);
 
@@ -33,17 +34,52 @@ int fail(int code, const std::string& msg) {
             return fail(PP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-struct PmtArgs {
-    float dir[kPmts][3];
-    float inv_s2[2];  // 1 / sigma^2 per group
-    int32_t n_wl;
-    const float* tab;  // wl | acc1 | acc2, n_wl each
+// Inputs of the detection kernel, one of two layouts:
+//   SoA      position relative to the module centre in radii, wavelength,
+//            photon id and source index per hit (pp_pom_pmt_hits)
+//   records  the context's hit records, module centre and radius from the
+//            target table (pp_pom_pmt_hits_ctx)
+struct PmtSoA {
     const float* rel_pos;
     const float* wl;
     const uint64_t* pid;
     const int32_t* src;
+    __device__ __forceinline__ void load(int64_t i, float& px, float& py, float& pz, float& w,
+                                         uint64_t& id, uint32_t& sx) const {
+        px = rel_pos[3 * i];
+        py = rel_pos[3 * i + 1];
+        pz = rel_pos[3 * i + 2];
+        w = wl[i];
+        id = pid[i];
+        sx = (uint32_t)src[i];
+    }
+};
+
+struct PmtRecords {
+    const pp::HitRecord* rec;
+    const float4* targets;  // x, y, z, radius
+    __device__ __forceinline__ void load(int64_t i, float& px, float& py, float& pz, float& w,
+                                         uint64_t& id, uint32_t& sx) const {
+        const float4* r = reinterpret_cast<const float4*>(rec + i);
+        const float4 a = r[0], c = r[2], d = r[3];
+        const uint64_t raw = (uint64_t)__float_as_uint(d.x) | ((uint64_t)__float_as_uint(d.y) << 32);
+        const float4 t = targets[((raw >> pp::kPidBits) & 0xFFFF) - 1];
+        // (position - centre) / radius, as the host computes it for a hit table
+        px = (a.x - t.x) / t.w;
+        py = (a.y - t.y) / t.w;
+        pz = (a.z - t.z) / t.w;
+        w = c.y;
+        id = raw & (pp::kMaxPhotonsPerSource - 1);
+        sx = __float_as_uint(d.w);
+    }
+};
+
+template <typename In>
+struct PmtArgs {
+    pp::PmtSetup s;
+    const float* tab;  // wl | acc1 | acc2, n_wl each
+    In in;
     int64_t n;
-    uint32_t k0, k1;
     int32_t* out;
     unsigned long long* n_det;
 };
@@ -64,10 +100,12 @@ __device__ __forceinline__ float interp0(const float* x, const float* y, int n, 
 // Grid-stride over the hits (a few workgroups per CU), so that the detected
 // count takes one atomic per workgroup: a wave ballot per pass, the waves'
 // sums in LDS at the end.
-__global__ void __launch_bounds__(256) pom_pmt_kernel(PmtArgs a) {
+template <typename In>
+__global__ void __launch_bounds__(256) pom_pmt_kernel(PmtArgs<In> a) {
     __shared__ float tab[3 * kMaxWl];
     __shared__ unsigned long long wave_det[4];
-    for (int i = threadIdx.x; i < 3 * a.n_wl; i += blockDim.x) tab[i] = a.tab[i];
+    const int n_wl = a.s.n_wl;
+    for (int i = threadIdx.x; i < 3 * n_wl; i += blockDim.x) tab[i] = a.tab[i];
     __syncthreads();
     unsigned long long n_det = 0;  // wave-uniform
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -75,28 +113,29 @@ __global__ void __launch_bounds__(256) pom_pmt_kernel(PmtArgs a) {
         const int64_t i = base + threadIdx.x;
         bool det = false;
         if (i < a.n) {
-            const float px = a.rel_pos[3 * i], py = a.rel_pos[3 * i + 1], pz = a.rel_pos[3 * i + 2];
-            const float wl = a.wl[i];
-            const float acc[2] = {interp0(tab, tab + a.n_wl, a.n_wl, wl),
-                                  interp0(tab, tab + 2 * a.n_wl, a.n_wl, wl)};
+            float px, py, pz, wl;
+            uint64_t id;
+            uint32_t sx;
+            a.in.load(i, px, py, pz, wl, id, sx);
+            const float acc[2] = {interp0(tab, tab + n_wl, n_wl, wl),
+                                  interp0(tab, tab + 2 * n_wl, n_wl, wl)};
             float p[kPmts], sum = 0.0f;
 #pragma unroll
             for (int j = 0; j < kPmts; ++j) {
                 const int g = (j >> 2) & 1;
-                const float c = fminf(fmaxf(px * a.dir[j][0] + py * a.dir[j][1] + pz * a.dir[j][2],
+                const float c = fminf(fmaxf(px * a.s.dir[j][0] + py * a.s.dir[j][1] +
+                                                pz * a.s.dir[j][2],
                                             -1.0f), 1.0f);
                 const float pt = acosf(c);
                 // sin(acos c) = sqrt((1 - c)(1 + c)), exact factors (no sinf)
                 const float st = __builtin_amdgcn_sqrtf((1.0f - c) * (1.0f + c));
                 // Rayleigh pdf over the density 0.5 sin(pt) of a uniform flux
-                const float pdf = pt * a.inv_s2[g] * __expf(-0.5f * pt * pt * a.inv_s2[g]);
+                const float pdf = pt * a.s.inv_s2[g] * __expf(-0.5f * pt * pt * a.s.inv_s2[g]);
                 p[j] = st != 0.0f ? pdf * (2.0f * __builtin_amdgcn_rcpf(st)) * acc[g] * 0.125f : 0.0f;
                 sum += p[j];
             }
-            const uint64_t id = a.pid[i];
-            const uint4 r = pp::philox(make_uint4(PP_CTR_PMT, (uint32_t)id, (uint32_t)(id >> 32),
-                                                  (uint32_t)a.src[i]),
-                                       a.k0, a.k1);
+            const uint4 r = pp::philox(make_uint4(PP_CTR_PMT, (uint32_t)id, (uint32_t)(id >> 32), sx),
+                                       a.s.k0, a.s.k1);
             int pmt = 0;
             if (pp::u01(r.x) < sum) {
                 // categorical draw: first j whose cumulative weight exceeds u * sum
@@ -122,6 +161,15 @@ __global__ void __launch_bounds__(256) pom_pmt_kernel(PmtArgs a) {
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wave_det[w];
         if (s) atomicAdd(a.n_det, s);
     }
+}
+
+template <typename In>
+int launch(const PmtArgs<In>& a, hipStream_t stream, int n_cu) {
+    const int64_t blocks = std::min<int64_t>((a.n + 255) / 256, (int64_t)std::max(n_cu, 1) * 8);
+    hipLaunchKernelGGL(pom_pmt_kernel<In>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256),
+                       0, stream, a);
+    PP_HIP(hipGetLastError());
+    return PP_OK;
 }
 
 // PMT (theta, phi) of the POM, src/pom.jl:108-128
@@ -157,6 +205,49 @@ int validate(const pp_pom_model* m) {
 
 }  // namespace
 
+namespace pp {
+
+int pmt_setup(const pp_pom_model* m, const float* wl_grid, const float* acc1, const float* acc2,
+              uint64_t seed, PmtSetup* s, float* tab) {
+    if (int rc = validate(m)) return rc;
+    if (!wl_grid || !acc1 || !acc2) return fail(PP_ERR_INVALID, "pom: NULL acceptance table");
+    for (int i = 1; i < m->n_wl; ++i)
+        if (!(wl_grid[i] > wl_grid[i - 1]))
+            return fail(PP_ERR_INVALID, "pom: wavelength grid must increase");
+    float dirs[kPmts * 3];
+    if (int rc = pp_pom_pmt_directions(m, dirs)) return rc;
+    for (int j = 0; j < kPmts; ++j)
+        for (int r = 0; r < 3; ++r) s->dir[j][r] = dirs[3 * j + r];
+    s->inv_s2[0] = 1.0f / (m->sigma_grp1 * m->sigma_grp1);
+    s->inv_s2[1] = 1.0f / (m->sigma_grp2 * m->sigma_grp2);
+    s->n_wl = m->n_wl;
+    s->k0 = (uint32_t)seed;
+    s->k1 = (uint32_t)(seed >> 32);
+    const size_t nw = (size_t)m->n_wl;
+    for (size_t k = 0; k < nw; ++k) {
+        tab[k] = wl_grid[k];
+        tab[nw + k] = acc1[k];
+        tab[2 * nw + k] = acc2[k];
+    }
+    return PP_OK;
+}
+
+int pmt_launch_records(const PmtSetup& s, const float* tab, const HitRecord* rec, int64_t n,
+                       const float4* targets, int32_t* out, unsigned long long* count,
+                       hipStream_t stream, int n_cu) {
+    PmtArgs<PmtRecords> a{};
+    a.s = s;
+    a.tab = tab;
+    a.in.rec = rec;
+    a.in.targets = targets;
+    a.n = n;
+    a.out = out;
+    a.n_det = count;
+    return launch(a, stream, n_cu);
+}
+
+}  // namespace pp
+
 extern "C" {
 
 int pp_pom_pmt_directions(const pp_pom_model* m, float* out) {
@@ -185,28 +276,17 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, con
     if (n_detected) *n_detected = 0;
     if (kernel_ms) *kernel_ms = 0.0f;
     if (n == 0) return PP_OK;
-    if (!wl_grid || !acc1 || !acc2 || !rel_pos3 || !wavelength || !photon_id || !source_ix ||
-        !pmt_id_out)
+    if (!rel_pos3 || !wavelength || !photon_id || !source_ix || !pmt_id_out)
         return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: NULL array");
-    for (int i = 1; i < m->n_wl; ++i)
-        if (!(wl_grid[i] > wl_grid[i - 1]))
-            return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: wavelength grid must increase");
+    PmtArgs<PmtSoA> a{};
+    std::vector<float> tab(3 * (size_t)m->n_wl);
+    if (int rc = pp::pmt_setup(m, wl_grid, acc1, acc2, seed, &a.s, tab.data())) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
         return fail(PP_ERR_INVALID, "pp_pom_pmt_hits: no such HIP device");
     PP_HIP(hipSetDevice(device));
     (void)hipGetLastError();
-    PmtArgs a{};
-    float dirs[kPmts * 3];
-    if (int rc = pp_pom_pmt_directions(m, dirs)) return rc;
-    for (int j = 0; j < kPmts; ++j)
-        for (int r = 0; r < 3; ++r) a.dir[j][r] = dirs[3 * j + r];
-    a.inv_s2[0] = 1.0f / (m->sigma_grp1 * m->sigma_grp1);
-    a.inv_s2[1] = 1.0f / (m->sigma_grp2 * m->sigma_grp2);
-    a.n_wl = m->n_wl;
     a.n = n;
-    a.k0 = (uint32_t)seed;
-    a.k1 = (uint32_t)(seed >> 32);
     const size_t nw = (size_t)m->n_wl;
     // one allocation: table | rel_pos | wl | pid | src | out | counter
     const size_t off_pos = (3 * nw * 4 + 255) & ~(size_t)255;
@@ -225,12 +305,6 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, con
     auto chk = [&](hipError_t e, const char* what) {
         if (e != hipSuccess && rc == PP_OK) rc = fail(PP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     };
-    std::vector<float> tab(3 * nw);
-    for (size_t k = 0; k < nw; ++k) {
-        tab[k] = wl_grid[k];
-        tab[nw + k] = acc1[k];
-        tab[2 * nw + k] = acc2[k];
-    }
     chk(hipMemcpy(d, tab.data(), 3 * nw * 4, hipMemcpyHostToDevice), "upload table");
     chk(hipMemcpy(d + off_pos, rel_pos3, (size_t)n * 12, hipMemcpyHostToDevice), "upload positions");
     chk(hipMemcpy(d + off_wl, wavelength, (size_t)n * 4, hipMemcpyHostToDevice), "upload wavelengths");
@@ -239,10 +313,10 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, con
     chk(hipMemset(d + off_cnt, 0, 8), "clear counter");
     if (rc == PP_OK) {
         a.tab = (const float*)d;
-        a.rel_pos = (const float*)(d + off_pos);
-        a.wl = (const float*)(d + off_wl);
-        a.pid = (const uint64_t*)(d + off_pid);
-        a.src = (const int32_t*)(d + off_src);
+        a.in.rel_pos = (const float*)(d + off_pos);
+        a.in.wl = (const float*)(d + off_wl);
+        a.in.pid = (const uint64_t*)(d + off_pid);
+        a.in.src = (const int32_t*)(d + off_src);
         a.out = (int32_t*)(d + off_out);
         a.n_det = (unsigned long long*)(d + off_cnt);
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -255,10 +329,7 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, con
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        const int64_t blocks = std::min<int64_t>((n + 255) / 256, (int64_t)cus * 8);
-        if (rc == PP_OK)
-            hipLaunchKernelGGL(pom_pmt_kernel, dim3((unsigned)blocks), dim3(256), 0, 0, a);
-        chk(hipGetLastError(), "pom_pmt_kernel launch");
+        if (rc == PP_OK) rc = launch(a, 0, cus);
         if (kernel_ms && rc == PP_OK) {
             chk(hipEventRecord(e1, 0), "event record");
             chk(hipEventSynchronize(e1), "event sync");

@@ -30,6 +30,16 @@ struct GridDesc {
     int nx, ny, nz;
     int n_cells, n_items;
     float zpad;        // largest target radius plus the member boxes' margin
+    float ex, ey, ez;  // upper corner, ox + nx h etc. (rounded once, on the host)
+};
+
+// The step segment clipped to the grid box: reciprocal direction components,
+// plane offsets and the parameter interval [t_in, t_out] inside the box
+// (empty if t_in > t_out). Every operation is spelled out (explicit FMAs,
+// products that cannot fuse), so two evaluations with the same inputs give
+// the same bits wherever the compiler inlines them.
+struct Clip {
+    float ix, iy, iz, px, py, pz, t_in, t_out;
 };
 
 // Work done by one search (instrumented builds only).
@@ -155,24 +165,39 @@ struct GridSearch {
         return g.nz == 1 ? walk<COUNT, true>(pos, dir, step, dt, w)
                          : walk<COUNT, false>(pos, dir, step, dt, w);
     }
-    template <bool COUNT, bool COLS>
-    __host__ __device__ __forceinline__ int walk(f3 pos, f3 dir, float step, float& dt,
-                                                 Work& w) const {
+    __host__ __device__ __forceinline__ Clip clip(f3 pos, f3 dir, float step) const {
+        Clip c;
         // reciprocal direction components, finite (|1/d| <= 1e20, the sign
-        // of a zero component kept); the walk's step signs below come from
-        // these same values so that they agree with ix_..iz_
-        const float ix_ = rcp_clamped(dir.x), iy_ = rcp_clamped(dir.y), iz_ = rcp_clamped(dir.z);
+        // of a zero component kept); the walk's step signs come from these
+        // same values so that they agree with ix..iz
+        c.ix = rcp_clamped(dir.x);
+        c.iy = rcp_clamped(dir.y);
+        c.iz = rcp_clamped(dir.z);
         // plane parameters as one FMA per plane: (b - p) / d = b (1/d) - p (1/d);
         // the rounding of p (1/d) moves a plane by about |p| 2^-24 along its
         // axis, far inside the binning margin and the boxes' 1 cm margin
-        const float px_ = pos.x * ix_, py_ = pos.y * iy_, pz_ = pos.z * iz_;
+        c.px = pos.x * c.ix;
+        c.py = pos.y * c.iy;
+        c.pz = pos.z * c.iz;
         // clip the segment [0, step] to the grid box
-        const float ex = g.ox + g.nx * g.h, ey = g.oy + g.ny * g.h, ez = g.oz + g.nz * g.hz;
-        const float tx0 = fmaf(g.ox, ix_, -px_), tx1 = fmaf(ex, ix_, -px_);
-        const float ty0 = fmaf(g.oy, iy_, -py_), ty1 = fmaf(ey, iy_, -py_);
-        const float tz0 = fmaf(g.oz, iz_, -pz_), tz1 = fmaf(ez, iz_, -pz_);
-        const float t_in = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        const float t_out = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), step));
+        const float tx0 = fmaf(g.ox, c.ix, -c.px), tx1 = fmaf(g.ex, c.ix, -c.px);
+        const float ty0 = fmaf(g.oy, c.iy, -c.py), ty1 = fmaf(g.ey, c.iy, -c.py);
+        const float tz0 = fmaf(g.oz, c.iz, -c.pz), tz1 = fmaf(g.ez, c.iz, -c.pz);
+        c.t_in = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        c.t_out = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), step));
+        return c;
+    }
+    template <bool COUNT, bool COLS>
+    __host__ __device__ __forceinline__ int walk(f3 pos, f3 dir, float step, float& dt,
+                                                 Work& w) const {
+        return walk_from<COUNT, COLS>(clip(pos, dir, step), pos, dir, step, dt, w);
+    }
+    // the walk over the clipped segment; cl = clip(pos, dir, step)
+    template <bool COUNT, bool COLS>
+    __host__ __device__ __forceinline__ int walk_from(const Clip& cl, f3 pos, f3 dir, float step,
+                                                      float& dt, Work& w) const {
+        const float ix_ = cl.ix, iy_ = cl.iy, iz_ = cl.iz, px_ = cl.px, py_ = cl.py, pz_ = cl.pz;
+        const float t_in = cl.t_in, t_out = cl.t_out;
         int found = -1;
         if (!(t_in <= t_out)) return found;
         // entry cell

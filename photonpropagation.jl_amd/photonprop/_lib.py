@@ -64,7 +64,7 @@ FN = dict(ISOTROPIC_DIR=0, ROT_FROM_EZ=1, HG_COS=2, ES_COS=3, UPDATE_DIR=4, SPHE
           STEP_LEN=6, CASCADIA=7, TRACK_DIR=8, PHILOX=9, GAMMA=10, SPECTRUM=11, SL_COS=12,
           PLANE_ISEC=13, MIXED_HGES=14)
 FN_ARITY = {0: (2, 3), 1: (6, 3), 2: (2, 1), 3: (2, 1), 4: (5, 3), 5: (11, 2), 6: (2, 1),
-            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 1), 11: (1, 1), 12: (2, 1),
+            7: (1, 3), 8: (2, 3), 9: (6, 4), 10: (7, 2), 11: (1, 1), 12: (2, 1),
             13: (13, 2), 14: (5, 1)}
 
 
@@ -160,6 +160,7 @@ SIGNATURES = [
                              c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p]),
     ("pp_last_timing", c_int, [c_void_p, POINTER(c_float), POINTER(c_int32)]),
+    ("pp_release_scratch", c_int, [c_void_p]),
     ("pp_stream", c_void_p, [c_void_p]),
     ("pp_search_info", c_int, [c_void_p, POINTER(c_int32), POINTER(c_float), POINTER(c_int32),
                                POINTER(c_int32)]),
@@ -168,6 +169,9 @@ SIGNATURES = [
     ("pp_pom_pmt_hits", c_int, [c_int, POINTER(pp_pom_model), c_void_p, c_void_p, c_void_p,
                                 c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_uint64,
                                 c_void_p, POINTER(c_int64), POINTER(c_float)]),
+    ("pp_pom_pmt_hits_ctx", c_int, [c_void_p, POINTER(pp_pom_model), c_void_p, c_void_p,
+                                    c_void_p, c_uint64, c_int64, c_int64, c_void_p,
+                                    POINTER(c_int64), POINTER(c_float)]),
     ("pp_device_info", c_int, [c_int, ctypes.c_char_p, c_int, POINTER(c_int32)]),
     ("pp_microbench", c_int, [c_int, c_int, POINTER(ctypes.c_double)]),
     ("pp_get_config", c_int, [c_void_p, POINTER(pp_config)]),
@@ -190,6 +194,8 @@ SIGNATURES = [
     ("pp_sur_eval_pairs", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int,
                                   POINTER(pp_sur_domain), c_void_p, c_void_p, c_void_p,
                                   POINTER(c_float)]),
+    ("pp_sur_hit_labels", c_int, [c_void_p, c_void_p, c_void_p, c_int, ctypes.c_double,
+                                  c_void_p, c_int, c_void_p]),
     ("pp_autotune", c_int, [c_void_p, c_void_p, c_int, c_uint64, c_int32, c_float, c_int64,
                             POINTER(pp_config), POINTER(c_float)]),
     ("pp_eval_fn", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int64]),

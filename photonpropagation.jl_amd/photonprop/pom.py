@@ -4,7 +4,7 @@ A photon that reached a module's sphere is turned into a PMT hit (1..16) or
 dropped by pp_pom_pmt_hits on the GPU: per-PMT Rayleigh angular acceptance
 around each PMT's direction times the wavelength acceptance of its PMT group
 (assets/pmt_acc.hd5 of the reference, converted to data/pom_acceptance.csv by
-data/convert_pom_acceptance.py). Its random draws come from the photon's own
+tests/golden/convert_pom_acceptance.py). Its random draws come from the photon's own
 Philox stream, so a photon's PMT does not depend on how hits are batched.
 """
 from __future__ import annotations
@@ -134,6 +134,30 @@ def _pmt_ids(rel_pos, wl, acc, orientation, photon_id, source_ix, seed, device, 
     _lib.check(_lib.load_library().pp_pom_pmt_hits(
         device, ctypes.byref(m), _p(grid), _p(a1), _p(a2), n, _p(rp), _p(w), _p(pid), _p(src),
         ctypes.c_uint64(seed), _p(out), ctypes.byref(nd), ctypes.byref(ms)), "pp_pom_pmt_hits")
+    if timing is not None:
+        timing["kernel_ms"] = float(ms.value)
+        timing["n_detected"] = int(nd.value)
+    return out
+
+
+def pmt_ids_in_context(dev, acceptance: POMAcceptance | None = None, orientation=None, *,
+                       seed=0, first=0, n=None, timing=None):
+    """PMT ids of the hits of the context's last propagation, computed on the
+    device from its hit buffer (pp_pom_pmt_hits_ctx): rows [first, first + n)
+    in the order of run_photon_prop_no_local_cache's columns."""
+    lib = _lib.load_library()
+    acc = acceptance or default_acceptance()
+    count = int(dev.n_hits) - int(first) if n is None else int(n)
+    grid = np.ascontiguousarray(acc.wavelengths, np.float32)
+    a1 = np.ascontiguousarray(acc.acc_1, np.float32)
+    a2 = np.ascontiguousarray(acc.acc_2, np.float32)
+    out = np.zeros(max(count, 0), np.int32)
+    nd = ctypes.c_int64()
+    ms = ctypes.c_float()
+    m = _model(acc, orientation)
+    _lib.check(lib.pp_pom_pmt_hits_ctx(dev.handle, ctypes.byref(m), _p(grid), _p(a1), _p(a2),
+                                       ctypes.c_uint64(seed), int(first), len(out), _p(out),
+                                       ctypes.byref(nd), ctypes.byref(ms)), "pp_pom_pmt_hits_ctx")
     if timing is not None:
         timing["kernel_ms"] = float(ms.value)
         timing["n_detected"] = int(nd.value)

@@ -53,6 +53,7 @@ class DeviceHitBuffer:
                    "pp_create")
         self.handle = h
         self.config = cfg
+        self.n_hits = 0  # hits of the last propagation
 
     def close(self):
         if self.handle:
@@ -76,6 +77,11 @@ class DeviceHitBuffer:
         n = ctypes.c_int32()
         _lib.load_library().pp_last_timing(self.handle, ctypes.byref(ms), ctypes.byref(n))
         return ms.value, n.value
+
+    def release_scratch(self):
+        """Free the buffers the context grows on demand (ordered-copy sort
+        buffer, pinned copy staging, PMT detection buffers)."""
+        _lib.check(_lib.load_library().pp_release_scratch(self.handle), "pp_release_scratch")
 
 
 @dataclass
@@ -146,6 +152,7 @@ def run_photon_prop_no_local_cache(sources, targets, medium, spectrum, seed,
         raise PhotonPropOOMException()
     _lib.check(rc, "pp_propagate")
     n = n_hits.value
+    dev.n_hits = n
     ttype = np.float64 if hit_buffer_cpu.time_type in (np.float64, float) else np.float32
     cols = dict(position=np.empty((n, 3), np.float32), direction=np.empty((n, 3), np.float32),
                 initial_direction=np.empty((n, 3), np.float32),

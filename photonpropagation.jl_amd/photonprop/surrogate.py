@@ -29,8 +29,9 @@ from . import _lib
 from .detection import HomogeneousDetector, Spherical
 from .lightyield import (ExtendedCherenkovEmitter, LongitudinalParameterisation, Particle,
                          PEMinus, total_lightyield)
-from .medium import CascadiaMediumProperties
-from .processing import (HostHitBuffer, make_hit_buffers, run_photon_prop_no_local_cache)
+from .medium import STRAW_ABS_LEN, STRAW_WL, CascadiaMediumProperties
+from .processing import (HostHitBuffer, make_hit_buffers, run_photon_prop_no_local_cache,
+                         sources_to_pod, stage)
 
 N_IN = _lib.SUR_N_FEATURES
 N_OUT = _lib.SUR_N_OUT
@@ -233,13 +234,17 @@ def photons_for_distance(r, lo=2e4, hi=4e6, scale=2e4):
 
 
 def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sources=2048,
-                           photons_fn=photons_for_distance, dev=None, log=print):
+                           photons_fn=photons_for_distance, dev=None, log=print,
+                           r_range=(1.0, 800.0), device_labels=True):
     """Labels from the Monte Carlo: every sample is a cascade (vertex, axis,
     energy) around one module at the origin; batches of samples run as the
-    sources of one persistent propagation launch. Returns (cascades (n, 8),
-    labels (n, 66), stats)."""
+    sources of one persistent propagation launch. With device_labels the
+    hits are reduced to labels on the device (pp_sur_hit_labels: only the
+    labels come back); otherwise the hits are copied and reduced here.
+    Returns (cascades (n, 8), labels (n, 66), stats)."""
+    lib = _lib.load_library()
     rng = np.random.default_rng(seed)
-    pos, d, e = sample_geometry(rng, n_samples)
+    pos, d, e = sample_geometry(rng, n_samples, r_range=r_range)
     r = np.linalg.norm(pos, axis=1)
     nmc = photons_fn(r)
     logy = setup.log_yield(e)
@@ -249,34 +254,69 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
     labels[:, 0] = exposure(nmc, r, setup.module_radius)
     own = dev is None
     if own:
-        _, dev = make_hit_buffers(capacity=1 << 25)
+        _, dev = make_hit_buffers(capacity=1 << 26)
     c_ref = setup.c_ref
+    medium = setup.medium
+    abs_wl = np.ascontiguousarray(STRAW_WL, np.float64)
+    abs_len = np.ascontiguousarray(STRAW_ABS_LEN, np.float64)
     t0 = time.perf_counter()
     photons = 0
     kernel_ms = 0.0
+
+    def sources(b0, b1):
+        return [ExtendedCherenkovEmitter(pos[i], d[i], 0.0, int(nmc[i]),
+                                         LongitudinalParameterisation.from_energy(
+                                             float(e[i]), medium, PEMinus))
+                for i in range(b0, b1)]
+
+    def device_batch(b0, b1, key):
+        """Propagate samples [b0, b1) and reduce their hits on the device; a
+        batch whose hits overflow the buffer is run as two halves."""
+        nonlocal photons, kernel_ms
+        pods = sources_to_pod(sources(b0, b1))
+        nh, ns = ctypes.c_int64(), ctypes.c_int64()
+        rc = lib.pp_propagate(dev.handle, pods, b1 - b0, ctypes.c_uint64(key), setup.n_steps, 1.0,
+                              ctypes.byref(nh), ctypes.byref(ns))
+        if rc == _lib.PP_ERR_HIT_OVERFLOW and b1 - b0 > 1:
+            mid = (b0 + b1) // 2
+            device_batch(b0, mid, key)
+            device_batch(mid, b1, key + 1)
+            return
+        _lib.check(rc, "pp_propagate")
+        kernel_ms += dev.last_timing()[0]
+        photons += ns.value
+        tg = np.ascontiguousarray((r[b0:b1] - setup.module_radius) / c_ref, np.float64)
+        out = np.zeros((b1 - b0, 1 + BINS), np.float32)
+        _lib.check(lib.pp_sur_hit_labels(dev.handle, _p(abs_wl), _p(abs_len), len(abs_wl),
+                                         float(medium.abs_scale), _p(tg), b1 - b0, _p(out)),
+                   "pp_sur_hit_labels")
+        labels[b0:b1, 1:] = out
+
     try:
+        if device_labels:
+            stage(dev, [target], medium, setup.device_spectrum)
         for b0 in range(0, n_samples, batch_sources):
             b1 = min(n_samples, b0 + batch_sources)
-            srcs = [ExtendedCherenkovEmitter(pos[i], d[i], 0.0, int(nmc[i]),
-                                             LongitudinalParameterisation.from_energy(
-                                                 float(e[i]), setup.medium, PEMinus))
-                    for i in range(b0, b1)]
-            hits, nsim = run_photon_prop_no_local_cache(
-                srcs, [target], setup.medium, setup.device_spectrum, seed * 1000003 + b0,
-                HostHitBuffer(np.float32), dev, n_steps=setup.n_steps, with_ids=True)
-            kernel_ms += dev.last_timing()[0]
-            photons += nsim
-            s = hits["source_ix"].astype(np.int64) + b0
-            w = np.exp(-hits["dist_travelled"].astype(np.float64) /
-                       setup.medium.absorption_length(hits["wavelength"].astype(np.float64)))
-            labels[:, 1] += np.bincount(s, weights=w, minlength=n_samples).astype(np.float32)
-            tgeo = (r[s] - setup.module_radius) / c_ref
-            x = hits["time"].astype(np.float64) - tgeo + SHIFT
-            binx = np.floor(x / (TMAX / BINS)).astype(np.int64)
-            ok = (binx >= 0) & (binx < BINS)
-            flat = np.bincount((s[ok] - b0) * BINS + binx[ok], weights=w[ok],
-                               minlength=(b1 - b0) * BINS)
-            labels[b0:b1, 2:] += flat.reshape(b1 - b0, BINS).astype(np.float32)
+            if device_labels:
+                device_batch(b0, b1, seed * 1000003 + b0)
+            else:
+                hits, nsim = run_photon_prop_no_local_cache(
+                    sources(b0, b1), [target], medium, setup.device_spectrum,
+                    seed * 1000003 + b0, HostHitBuffer(np.float32), dev, n_steps=setup.n_steps,
+                    with_ids=True)
+                kernel_ms += dev.last_timing()[0]
+                photons += nsim
+                s = hits["source_ix"].astype(np.int64) + b0
+                w = np.exp(-hits["dist_travelled"].astype(np.float64) /
+                           medium.absorption_length(hits["wavelength"].astype(np.float64)))
+                labels[:, 1] += np.bincount(s, weights=w, minlength=n_samples).astype(np.float32)
+                tgeo = (r[s] - setup.module_radius) / c_ref
+                x = hits["time"].astype(np.float64) - tgeo + SHIFT
+                binx = np.floor(x / (TMAX / BINS)).astype(np.int64)
+                ok = (binx >= 0) & (binx < BINS)
+                flat = np.bincount((s[ok] - b0) * BINS + binx[ok], weights=w[ok],
+                                   minlength=(b1 - b0) * BINS)
+                labels[b0:b1, 2:] += flat.reshape(b1 - b0, BINS).astype(np.float32)
             if log and (b0 // batch_sources) % 10 == 0:
                 log(f"  labels {b1}/{n_samples}: {photons:.3g} photons, "
                     f"{time.perf_counter() - t0:.1f} s")
@@ -392,7 +432,7 @@ def mc_expected_hits(setup: SurrogateSetup, events, targets, *, max_photons=1e9,
         if own:
             dev.close()
     n_eff = np.where(w2 > 0, exp_hits ** 2 / np.maximum(w2, 1e-300), 0.0)
-    out = dict(hits=exp_hits, raw=raw, n_eff=n_eff, seconds=time.perf_counter() - t0,
+    out = dict(hits=exp_hits, raw=raw, n_eff=n_eff, w2=w2, seconds=time.perf_counter() - t0,
                kernel_ms=kernel_ms, photons=photons)
     if with_times:
         out["times"] = tuple(np.concatenate(c) for c in zip(*times)) if times else None
@@ -440,6 +480,54 @@ class SurrogateModel:
         return out
 
 
+DIST_EDGES = (5.0, 20.0, 50.0, 100.0, 200.0, 400.0, 800.0)
+HIT_EDGES = (0.0, 1.0, 10.0, 100.0, 1e3, 1e4, np.inf)
+
+
+def judge_strata(sur_hits, mc, fallback, dist, min_eff=100.0, min_raw=10):
+    """Every in-domain (event, module) pair, stratified by distance and by the
+    Monte Carlo's expected hits. Per stratum: pairs, pairs with MC hits, the
+    ratio of summed surrogate to summed MC expected hits (a bias), the pulls
+    (sur - mc) / sigma_mc over pairs with >= min_raw MC hits (sigma_mc =
+    sqrt(sum w^2), the MC's own statistical error: a pull RMS near 1 means the
+    surrogate is as close as the MC's statistics can tell), the mean relative
+    error over pairs whose MC estimate has >= min_eff effective hits, and the
+    surrogate's summed expectation on pairs the MC never hit."""
+    ind = ~fallback
+    mh, raw, w2, n_eff = mc["hits"], mc["raw"], mc["w2"], mc["n_eff"]
+    sigma = np.sqrt(w2)
+
+    def stats(sel):
+        d = dict(pairs=int(sel.sum()), pairs_with_hits=int((sel & (raw > 0)).sum()))
+        if not sel.any():
+            return d
+        tot_mc = float(mh[sel].sum())
+        d["sum_ratio"] = float(sur_hits[sel].sum() / tot_mc) if tot_mc > 0 else None
+        pull_sel = sel & (raw >= min_raw)
+        if pull_sel.any():
+            pull = (sur_hits[pull_sel] - mh[pull_sel]) / sigma[pull_sel]
+            d.update(pull_pairs=int(pull_sel.sum()), pull_mean=float(pull.mean()),
+                     pull_rms=float(np.sqrt(np.mean(pull ** 2))))
+        rel_sel = sel & (n_eff >= min_eff)
+        if rel_sel.any():
+            d.update(rel_pairs=int(rel_sel.sum()),
+                     mean_rel_err=float(np.mean(np.abs(sur_hits[rel_sel] / mh[rel_sel] - 1))))
+        zero = sel & (raw == 0)
+        if zero.any():
+            d["zero_hit_pairs"] = int(zero.sum())
+            d["sur_hits_on_zero_pairs"] = float(sur_hits[zero].sum())
+        return d
+
+    out = {"by_distance": [], "by_mc_hits": []}
+    for lo, hi in zip(DIST_EDGES[:-1], DIST_EDGES[1:]):
+        out["by_distance"].append(dict(range_m=[lo, hi], **stats(ind & (dist >= lo) & (dist < hi))))
+    for lo, hi in zip(HIT_EDGES[:-1], HIT_EDGES[1:]):
+        sel = ind & (mh >= lo) & (mh < hi) & (raw > 0)
+        out["by_mc_hits"].append(dict(range=[lo, None if np.isinf(hi) else hi], **stats(sel)))
+    out["all"] = stats(ind)
+    return out
+
+
 def judge_events(model: SurrogateModel, events, targets, *, max_photons=1e9, min_eff_hits=2500,
                  seed=21, dev=None):
     """The full Monte Carlo judges the surrogate on held-out events: mean
@@ -462,6 +550,9 @@ def judge_events(model: SurrogateModel, events, targets, *, max_photons=1e9, min
                mc_seconds=mc["seconds"], mc_kernel_ms=mc["kernel_ms"], mc_photons=mc["photons"],
                surrogate_ms=sur["surrogate_ms"], n_events=int(events.shape[0]),
                n_modules=len(targets))
+    mxyz = np.array([t.shape.position for t in targets], np.float64)
+    dist = np.linalg.norm(mxyz[None, :, :] - events[:, None, :3].astype(np.float64), axis=2)
+    res["strata"] = judge_strata(sur["hits"], mc, sur["fallback"], dist)
     if mc.get("times") is not None:
         ev, mod, x, w = mc["times"]
         keep = sel[ev, mod] & (x > 0) & (x < TMAX)

@@ -1,5 +1,6 @@
 """Convert the POM acceptance asset of the reference (assets/pmt_acc.hd5, read
-by POMAcceptance, src/pom.jl:25-35) into pom_acceptance.csv beside this file.
+by POMAcceptance, src/pom.jl:25-35) into the package's data file
+photonpropagation.jl_amd/photonprop/data/pom_acceptance.csv.
 
 h5py is not available here, so this reads the file's few datasets directly:
 HDF5 superblock version 0, version-1 object headers (with continuation
@@ -88,7 +89,9 @@ def main(path):
     wl = ds["wavelengths"][1]
     a1, a2 = ds["acc_pmt_grp_1"][1], ds["acc_pmt_grp_2"][1]
     s1, s2 = ds["sigma_grp_1"][1][0], ds["sigma_grp_2"][1][0]
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pom_acceptance.csv")
+    repo = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    out = os.path.join(repo, "photonpropagation.jl_amd", "photonprop", "data",
+                       "pom_acceptance.csv")
     with open(out, "w") as f:
         f.write(f"# POM acceptance from pmt_acc.hd5; sigma_grp_1={s1!r}; sigma_grp_2={s2!r}\n")
         f.write("wavelength,acc_pmt_grp_1,acc_pmt_grp_2\n")

@@ -117,3 +117,35 @@ def test_photon_table_hdf5(P, tmp_path):
         a = d["attrs"]
         assert a["mode"] == ("extended", "lightsabre_muon")[k - 1] and a["seed"] == 99 + k
         assert abs(a["distance"] - 20.0) < 1e-12 and "source_pos" in a
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_pmt_from_context_equals_host_entry(P, pom, order):
+    """pp_pom_pmt_hits_ctx reads the context's hit buffer on the device; its
+    PMT ids equal pp_pom_pmt_hits on the copied columns (relative positions
+    formed in Float32 as the kernel forms them), row for row in the order the
+    copy returns, for every copy order and for a sub-range."""
+    targets = [pom.POM(np.float32([x, y, z]), 100 + i) for i, (x, y, z) in enumerate(
+        [(0, 0, 10), (3, -2, 14), (-4, 1, 7), (1, 5, 12)])]
+    src = P.ExtendedCherenkovEmitter(np.float32([0, 0, 0]), np.float32([0, 0, 1]), 0.0,
+                                     4_000_000, P.LongitudinalParameterisation(7.837, 0.632, 0.356))
+    medium = P.CascadiaMediumProperties()
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = order
+    hbc, hbg = P.make_hit_buffers(capacity=1 << 22, config=cfg)
+    with hbg:
+        hits, _ = P.run_photon_prop_no_local_cache([src], targets, medium, P.Monochromatic(420.0),
+                                                   5, hbc, hbg, with_ids=True)
+        n = len(hits["wavelength"])
+        assert n > 2000
+        t = {}
+        ctx_ids = pom.pmt_ids_in_context(hbg, seed=9, timing=t)
+        sub = pom.pmt_ids_in_context(hbg, seed=9, first=n // 3, n=n // 4)
+    c = np.array([t_.shape.position for t_ in targets], np.float32)[hits["module_id"] - 1]
+    rel = (hits["position"] - c) / np.float32(0.3)
+    host = pom._pmt_ids(rel.astype(np.float32), hits["wavelength"], pom.default_acceptance(),
+                        None, hits["photon_id"], hits["source_ix"], 9, 0)
+    assert np.array_equal(ctx_ids, host)
+    assert np.array_equal(sub, host[n // 3:n // 3 + n // 4])
+    assert t["n_detected"] == int(np.sum(host > 0)) > 0
+    print(f"order {order}: {n} hits, {t['n_detected']} detected, kernel {t['kernel_ms']:.3f} ms")

@@ -27,7 +27,8 @@ def oracle_run(P, O, wl, photon_scaling=1.0, capacity=1 << 22):
                        photon_scaling=photon_scaling, capacity=capacity, **kw)
 
 
-def compare_photon_by_photon(g, r, max_diff_frac=0.005, tail_frac=0.02, module_diff_frac=0.0):
+def compare_photon_by_photon(g, r, max_diff_frac=0.005, tail_frac=0.02, module_diff_frac=0.0,
+                             init_tail=0.0):
     """Same photons hit the same module at the same step; emission values
     agree to round-off for all of them. Transport values agree to round-off
     (5 mm + 2e-5 of the path) for all but `tail_frac` of them: HG(0.95)
@@ -37,7 +38,11 @@ def compare_photon_by_photon(g, r, max_diff_frac=0.005, tail_frac=0.02, module_d
     agree within the module's diameter. With targets close to each other
     such a photon may reach another target instead: `module_diff_frac` of
     the common photons may then report a different module (and are left out
-    of the value comparison)."""
+    of the value comparison). `init_tail`: the fraction of initial
+    directions allowed beyond 2e-5 (all within 2e-3), for emitters that
+    draw a smearing angle as a Float32 cosine next to 1 (the Cherenkov
+    track: 1 - cos has density ~ x^-0.61 at 0, and one ulp of a cosine whose
+    sine is s moves the direction by 2^-24 / s)."""
     gk = g["source_ix"].astype(np.uint64) << np.uint64(40) | g["photon_id"]
     rk = r["source_ix"].astype(np.uint64) << np.uint64(40) | r["photon_id"]
     assert np.all(np.diff(gk.astype(np.float64)) > 0), "GPU hits not in (source, photon) order"
@@ -52,7 +57,10 @@ def compare_photon_by_photon(g, r, max_diff_frac=0.005, tail_frac=0.02, module_d
     assert np.all(g["module_id"][gi] == r["module_ix"][ri])
     assert np.all(g["n_steps"][gi] == r["n_steps"][ri])
     np.testing.assert_allclose(g["wavelength"][gi], r["wavelength"][ri], rtol=2e-6)
-    np.testing.assert_allclose(g["initial_direction"][gi], r["init_dir"][ri], atol=2e-5)
+    d_init = np.max(np.abs(g["initial_direction"][gi].astype(np.float64) - r["init_dir"][ri]),
+                    axis=1)
+    assert np.mean(d_init > 2e-5) <= init_tail and np.all(d_init < 2e-3), (
+        np.mean(d_init > 2e-5), d_init.max())
     dpos = np.max(np.abs(g["position"][gi] - r["pos"][ri]), axis=1)
     ddir = np.max(np.abs(g["direction"][gi] - r["dir"][ri]), axis=1)
     dt = np.abs(g["time"][gi] - r["time"][ri]) / np.maximum(r["time"][ri], 1.0)
@@ -120,7 +128,8 @@ def test_every_emitter_photon_parity(P, O, dev, k):
     g, gsim = gpu_run(P, dev, wl)
     r, rsim = oracle_run(P, O, wl)
     assert gsim == rsim == 300_000
-    n, d = compare_photon_by_photon(g, r)
+    track = isinstance(src, P.CherenkovTrackEmitter)
+    n, d = compare_photon_by_photon(g, r, init_tail=0.01 if track else 0.0)
     print(f"{type(src).__name__}: {n} common hits, {d} differing")
     assert n > 20
 
@@ -653,3 +662,38 @@ def test_microbench_issue_rates(P, gpu_lib):
     assert 0 < r[P._lib.MB_TRANS_F32] < r[P._lib.MB_FMA_F32] / 2
     assert 0 < r[P._lib.MB_MAD_U64] < r[P._lib.MB_FMA_F32] / 2
     assert gpu_lib.pp_microbench(0, 7, ctypes.byref(ctypes.c_double())) != 0
+
+
+def test_copy_of_millions_of_hits_through_bounded_staging(P, gpu_lib):
+    """More hits than one pinned staging chunk (2^20 records): the copies in
+    buffer order, device order and host order hold the same hits, the two
+    ordered copies agree row for row, and a copy after pp_release_scratch
+    (which frees the sort buffer and the staging) equals the one before."""
+    import ctypes
+    medium = P.SimpleMediumProperties(30.0, 1e9, 1.35, 0.9, 0.2)
+    src = P.PointlikeIsotropicEmitter(np.zeros(3, np.float32), 0.0, 12_000_000)
+    targets = [P.spherical_target((0.0, 0.0, 6.0), 5.0, 1)]
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 1
+    _, d = P.make_hit_buffers(capacity=1 << 22, config=cfg)
+    with d:
+        def copy(order):
+            cfg.deterministic_order = order
+            assert gpu_lib.pp_set_config(d.handle, ctypes.byref(cfg)) == 0
+            hits, _ = P.run_photon_prop_no_local_cache([src], targets, medium,
+                                                       P.Monochromatic(450.0), 3,
+                                                       P.HostHitBuffer(np.float32), d,
+                                                       with_ids=True)
+            return hits
+        h1 = copy(1)
+        n = len(h1["photon_id"])
+        print(f"{n} hits")
+        assert n > (1 << 21)
+        h0, h2 = copy(0), copy(2)
+        d.release_scratch()
+        h1b = copy(1)
+        for k in ("photon_id", "position", "time", "module_id"):
+            assert h1[k].tobytes() == h2[k].tobytes() == h1b[k].tobytes(), k
+        o = np.argsort(h0["photon_id"], kind="stable")
+        assert h0["photon_id"][o].tobytes() == h1["photon_id"].tobytes()
+        assert h0["position"][o].tobytes() == h1["position"].tobytes()
