@@ -361,6 +361,25 @@ int pp_pom_pmt_hits_ctx(pp_ctx* ctx, const pp_pom_model* model, const float* wl_
                         const float* acc1, const float* acc2, uint64_t seed, int64_t first,
                         int64_t n, int32_t* pmt_id_out, int64_t* n_detected, float* kernel_ms);
 
+/* pp_pom_pmt_hits on device arrays, allocating nothing: no counterpart in
+ * the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs (rel_pos3: n x 3, wavelength, photon_id, source_ix) and the output
+ * pmt_id_out (n int32) are device arrays of `device`; wl_grid, acc1, acc2 are
+ * host arrays of model->n_wl <= 1024 knots. The acceptance table and the
+ * detected-hit counter live in a static 12.25 KiB device array of the library
+ * (one per GPU, held until the process ends); n_wl > 1024 is rejected.
+ * Work runs on `stream` (a hipStream_t; NULL: the null stream); with
+ * n_detected not NULL the call waits for the stream and writes the count,
+ * otherwise it returns without waiting. Calls that overlap on one GPU share
+ * that array and race: serialise them (one stream, or one call at a time).
+ * Results equal pp_pom_pmt_hits's for the same hits and seed. */
+int pp_pom_pmt_hits_noalloc(int device, const pp_pom_model* model, const float* wl_grid,
+                            const float* acc1, const float* acc2, int64_t n,
+                            const float* rel_pos3, const float* wavelength,
+                            const uint64_t* photon_id, const int32_t* source_ix, uint64_t seed,
+                            int32_t* pmt_id_out, int64_t* n_detected, void* stream);
+
 /* Autotuner. The reference fixes its launch shape from the occupancy API
  * (src/photon_prop_cuda.jl:914-920); this library searches it instead.
  * Times pp_propagate on a sample of `sources` (photon counts scaled so that

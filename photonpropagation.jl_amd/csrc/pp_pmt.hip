@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -248,6 +249,44 @@ int pmt_launch_records(const PmtSetup& s, const float* tab, const HitRecord* rec
 
 }  // namespace pp
 
+namespace {
+
+// Scratch of pp_pom_pmt_hits_noalloc: the acceptance table (3 x kMaxWl
+// floats) and the detected-hit counter, in a file-scope device array that
+// each GPU the library runs on holds until the process ends. Host code uses
+// the per-device address from hipGetSymbolAddress, never the array's name.
+constexpr size_t kScratchTableBytes = 3 * kMaxWl * sizeof(float);
+constexpr size_t kScratchBytes = kScratchTableBytes + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// The scratch array's address on `device` (current), looked up once per
+// device on its first call.
+int scratch_address(int device, char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PP_ERR_HIP, std::string("pp_pom_pmt_hits_noalloc: hipGetSymbolAddress: ") +
+                                        hipGetErrorString(e));
+        g_scratch[device] = (char*)p;
+    }
+    *out = g_scratch[device];
+    return PP_OK;
+}
+
+// Clears the counter with a kernel (no memset node where a caller captures
+// the stream into a graph).
+__global__ void clear_count_kernel(unsigned long long* c) {
+    if (threadIdx.x == 0) *c = 0ull;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pp_pom_pmt_directions(const pp_pom_model* m, float* out) {
@@ -344,6 +383,59 @@ int pp_pom_pmt_hits(int device, const pp_pom_model* m, const float* wl_grid, con
     }
     (void)hipFree(d);
     return rc;
+}
+
+int pp_pom_pmt_hits_noalloc(int device, const pp_pom_model* m, const float* wl_grid,
+                             const float* acc1, const float* acc2, int64_t n,
+                             const float* d_rel_pos3, const float* d_wavelength,
+                             const uint64_t* d_photon_id, const int32_t* d_source_ix,
+                             uint64_t seed, int32_t* d_pmt_id_out, int64_t* n_detected,
+                             void* stream) {
+    if (n_detected) *n_detected = 0;
+    if (int rc = validate(m)) return rc;  // n_wl <= kMaxWl: the table fits the scratch
+    if (n < 0 || n > ((int64_t)1 << 31))
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_noalloc: n out of range");
+    if (n == 0) return PP_OK;
+    if (!d_rel_pos3 || !d_wavelength || !d_photon_id || !d_source_ix || !d_pmt_id_out)
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_noalloc: NULL array");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev ||
+        device >= kMaxDevices)
+        return fail(PP_ERR_INVALID, "pp_pom_pmt_hits_noalloc: no such HIP device");
+    PP_HIP(hipSetDevice(device));
+    (void)hipGetLastError();
+    char* sc = nullptr;
+    if (int rc = scratch_address(device, &sc)) return rc;
+    PmtArgs<PmtSoA> a{};
+    float tab[3 * kMaxWl];
+    if (int rc = pp::pmt_setup(m, wl_grid, acc1, acc2, seed, &a.s, tab)) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    auto* d_cnt = (unsigned long long*)(sc + kScratchTableBytes);
+    // from pageable host memory: HIP stages the bytes before the call returns,
+    // so the stack copy may go out of scope while the transfer is queued
+    PP_HIP(hipMemcpyAsync(sc, tab, 3 * (size_t)m->n_wl * sizeof(float), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(64), 0, st, d_cnt);
+    PP_HIP(hipGetLastError());
+    a.tab = (const float*)sc;
+    a.in.rel_pos = d_rel_pos3;
+    a.in.wl = d_wavelength;
+    a.in.pid = d_photon_id;
+    a.in.src = d_source_ix;
+    a.n = n;
+    a.out = d_pmt_id_out;
+    a.n_det = d_cnt;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (int rc = launch(a, st, cus)) return rc;
+    if (n_detected) {
+        unsigned long long cnt = 0;
+        PP_HIP(hipMemcpyAsync(&cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, st));
+        PP_HIP(hipStreamSynchronize(st));
+        *n_detected = (int64_t)cnt;
+    }
+    return PP_OK;
 }
 
 }  // extern "C"

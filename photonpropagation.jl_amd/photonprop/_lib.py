@@ -172,6 +172,10 @@ SIGNATURES = [
     ("pp_pom_pmt_hits_ctx", c_int, [c_void_p, POINTER(pp_pom_model), c_void_p, c_void_p,
                                     c_void_p, c_uint64, c_int64, c_int64, c_void_p,
                                     POINTER(c_int64), POINTER(c_float)]),
+    ("pp_pom_pmt_hits_noalloc", c_int, [c_int, POINTER(pp_pom_model), c_void_p, c_void_p,
+                                        c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_uint64, c_void_p, POINTER(c_int64),
+                                        c_void_p]),
     ("pp_device_info", c_int, [c_int, ctypes.c_char_p, c_int, POINTER(c_int32)]),
     ("pp_microbench", c_int, [c_int, c_int, POINTER(ctypes.c_double)]),
     ("pp_get_config", c_int, [c_void_p, POINTER(pp_config)]),
