@@ -43,12 +43,14 @@ W_PHYS_TRANSC = 7.4
 W_ISEC_FLOP = 20.0
 W_ISEC_TRANSC = 1.0
 # Grid search (the uniform grid that replaces the reference's every-target
-# scan): per step the segment is clipped to the grid box
-# (3 reciprocals, 6 slab products, 12 min/max, entry cell and boundary
-# parameters: 25 FP32 ops); per visited cell the walk compares and advances
-# (min3, 2 compares, 1 add: 4 FP32 ops).
-W_GRID_STEP_FLOP = 25.0
-W_GRID_STEP_TRANSC = 3.0
+# scan): per step the segment's end points are compared with the grid box
+# (6 min/max, 6 compares: 12 FP32 ops); a segment near the grid is walked:
+# clipped to the box (3 reciprocals, 6 slab products, 12 min/max, entry cell
+# and boundary parameters: 25 FP32 ops) and per visited cell the walk
+# compares and advances (min3, 2 compares, 1 add: 4 FP32 ops).
+W_GRID_STEP_FLOP = 12.0
+W_GRID_WALK_FLOP = 25.0
+W_GRID_WALK_TRANSC = 3.0
 W_CELL_FLOP = 4.0
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # without packed-FP32 instructions
@@ -580,11 +582,13 @@ def main():
     P._lib.check(lib.pp_propagate(cdev.handle, pods, len(pods), ctypes.c_uint64(wl.seed), wl.n_steps,
                                   1.0, ctypes.byref(n_hits), ctypes.byref(n_sim)),
                  "pp_propagate (count_work)")
-    w_steps, w_tests, w_cells = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    w_steps, w_tests, w_cells, w_walks = (ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(),
+                                          ctypes.c_int64())
     P._lib.check(lib.pp_work_counts(cdev.handle, ctypes.byref(w_steps), ctypes.byref(w_tests),
-                                    ctypes.byref(w_cells)), "pp_work_counts")
+                                    ctypes.byref(w_cells), ctypes.byref(w_walks)),
+                 "pp_work_counts")
     cdev.close()
-    steps, tests, cells = w_steps.value, w_tests.value, w_cells.value
+    steps, tests, cells, walks = w_steps.value, w_tests.value, w_cells.value, w_walks.value
     assert steps == total_steps(hits, nsim, wl.n_steps)
     n_t = len(wl.targets)
     # algorithmic work: the reference step plus the sphere tests executed
@@ -592,8 +596,9 @@ def main():
     transc = steps * W_PHYS_TRANSC + tests * W_ISEC_TRANSC
     # the grid's own clip and walk, reported apart (work of this build's
     # search, not of the reference algorithm)
-    flop_grid = (steps * W_GRID_STEP_FLOP + cells * W_CELL_FLOP) if grid else 0.0
-    transc_grid = steps * W_GRID_STEP_TRANSC if grid else 0.0
+    flop_grid = (steps * W_GRID_STEP_FLOP + walks * W_GRID_WALK_FLOP + cells * W_CELL_FLOP
+                 if grid else 0.0)
+    transc_grid = walks * W_GRID_WALK_TRANSC if grid else 0.0
     achieved = flop / (kms * 1e-3) / 1e12
     traffic, traffic_src = measured_traffic(wl.name)
     counters = measured_counters(wl.name)
@@ -629,6 +634,7 @@ def main():
         "hits_per_step": int(len(hits["n_steps"])),
         "sphere_tests_per_step": tests / steps,
         "cell_visits_per_step": cells / steps,
+        "grid_walks_per_step": walks / steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic,
@@ -641,8 +647,9 @@ def main():
                      "grid_overhead": {"flop_per_launch": flop_grid,
                                        "tflops": flop_grid / (kms * 1e-3) / 1e12,
                                        "transcendental_per_s": transc_grid / (kms * 1e-3),
-                                       "basis": "25 FP32 ops + 3 reciprocals per step (clip), "
-                                                "4 per cell visit (walk)"},
+                                       "basis": "12 FP32 ops per step (box test of the end "
+                                                "points), 25 FP32 ops + 3 reciprocals per walk "
+                                                "(clip), 4 per cell visit"},
                      "basis": "photon steps x (47 + 20 T) FP32 ops, T = sphere tests executed "
                               "per step, counted by the instrumented kernel (identical hits); "
                               "transcendentals 7.4 + T per step",

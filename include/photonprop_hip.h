@@ -402,10 +402,13 @@ int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t s
                 int32_t n_steps, float photon_scaling, int64_t sample_photons,
                 pp_config* best, float* best_ms);
 
-/* Photon steps, sphere tests and grid-cell visits executed by the last
+/* Photon steps, sphere tests, grid-cell visits and grid walks (step
+ * segments near the grid that were clipped and walked) executed by the last
  * pp_propagate of a context created with cfg->count_work (roofline
- * accounting; the hits are identical to the uninstrumented kernel's). */
-int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* cell_visits);
+ * accounting; the hits are identical to the uninstrumented kernel's). Any
+ * output may be NULL. */
+int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* cell_visits,
+                   int64_t* grid_walks);
 
 /* ---- step-function evaluation (parity tests) ----------------------------
  * Evaluates one device step function over n explicit inputs on the GPU,

@@ -50,7 +50,7 @@ struct Counters {
     unsigned int err;             // 1 = overflow
     unsigned int pad;
     // work counts of the instrumented kernel (pp_config.count_work)
-    unsigned long long steps, tests, cells;
+    unsigned long long steps, tests, cells, walks;
     unsigned long long next_chunk;  // work-stealing counter of the batch kernel
 };
 
@@ -332,18 +332,20 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
 
 // ---------------------------------------------------------------------------
 // Deferred walk of the grid searches. A wave transports its 64 photons step
-// by step as in transport_photon, but a step whose segment meets the grid box
-// does not walk the grid at once: the segment becomes a job in the wave's LDS
-// queue, the photon moves on as if nothing were hit, and whenever 64 jobs are
-// queued every lane of the wave walks one of them. The walks of a step run
-// with 20-30% of the lanes in the old schedule (the others are outside the
-// grid or done); in a batch every lane walks. A job that hits records
-// (step << 16 | target) in its photon's LDS slot with an atomic minimum; the
-// earliest step wins, which is the step at which transport_photon stops (its
-// later steps were never taken by the reference and their jobs only lose the
-// minimum). At the end the few photons with a hit replay their steps up to
-// it (same random stream, same arithmetic) for the hit record. The result is
-// that of transport_photon bit for bit.
+// by step as in transport_photon, but a step whose segment comes near the
+// grid box (its end points' box meets it) does not walk the grid at once: the
+// segment becomes a job in the wave's LDS queue, the photon moves on as if
+// nothing were hit, and whenever 64 jobs are queued every lane of the wave
+// walks one of them. The walks of a step run with 20-30% of the lanes in the
+// one-photon schedule (the others are outside the grid or done); in a batch
+// every lane walks. A job that hits records (step << 16 | target) in its
+// photon's LDS slot with an atomic minimum; the earliest step wins, which is
+// the step at which transport_photon stops (its later steps were never taken
+// by the reference and their jobs only lose the minimum); a photon stops
+// stepping once a hit of its own is known. At the end the few photons with a
+// hit are emitted again and replay their steps up to it (same random stream,
+// same arithmetic) for the hit record; no emission state stays live across
+// the steps. The result is that of transport_photon bit for bit.
 // ---------------------------------------------------------------------------
 // Jobs per wave: a batch of up to 64 runs at the start of a step whenever 64
 // or more are pending or the step's new jobs would not fit, so at most kQueue
@@ -385,11 +387,16 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
                                                 HitRec& h) {
     const uint32_t lane = __lane_id();
     pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), src_ix};
-    PhotonInit st{};
-    if (active) st = init_photon(p, src, spec, s);
+    f3 pos = pp::make3(0.0f, 0.0f, 0.0f), dir = pos;
+    float ln2_sca = 0.0f;
+    if (active) {
+        // only what the steps need stays live; a hit emits the photon again
+        const PhotonInit st = init_photon(p, src, spec, s);
+        pos = st.pos;
+        dir = st.dir;
+        ln2_sca = st.mv.sca_len * pp::kLn2;
+    }
     q.key[lane] = ~0ull;
-    f3 pos = st.pos, dir = st.dir;
-    const float ln2_sca = st.mv.sca_len * pp::kLn2, inv_c = st.mv.inv_c_grp;
     // going: still stepping (active, and no hit found so far; steps after a
     // found hit cannot change the earliest one)
     bool going = active, stop = false;
@@ -407,6 +414,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
             const float4 a = q.job[2 * slot], b = q.job[2 * slot + 1];
             const uint32_t meta = __float_as_uint(b.w);
             owner = meta & 63u;
+            if (COUNT) work.walks += 1;
             found = search.template find<COUNT>(pp::make3(a.x, a.y, a.z), pp::make3(b.x, b.y, b.z),
                                                 a.w, dt, work);
             if (found >= 0) {
@@ -434,12 +442,14 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         if (stepping) ++step;
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
         float step_size = 0.0f;
+        f3 next = pos;
         bool need = false;
         if (move) {
             r = s.block((uint32_t)step);
             step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
-            const pp::Clip c = search.clip(pos, dir, step_size);
-            need = c.t_in <= c.t_out;
+            next = add_scaled(pos, step_size, dir);
+            // a walk only for a segment near the grid; the walk clips it exactly
+            need = search.touches(pos, next);
         }
         const uint64_t m = __ballot(need);
         const uint32_t n_new = (uint32_t)__popcll(m);
@@ -455,7 +465,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         }
         count += n_new;
         if (move) {
-            pos = add_scaled(pos, step_size, dir);
+            pos = next;
             const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
             dir = pp::update_direction(dir, ct, pp::u01(r.w));
         }
@@ -465,7 +475,9 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
     const bool hit = active && key != ~0ull;
     if (COUNT && active) n_steps_done += hit ? (unsigned long long)(key >> 16) : (unsigned long long)p.n_steps;
     if (hit) {
-        // the photon's steps again, up to the one that hits
+        // the photon's emission and steps again, up to the one that hits
+        const PhotonInit st = init_photon(p, src, spec, s);
+        const float inv_c = st.mv.inv_c_grp;
         const int32_t k = (int32_t)(key >> 16);
         const float dtk = q.dt[lane];
         f3 rp = st.pos, rd = st.dir;
@@ -491,6 +503,7 @@ __device__ __forceinline__ void flush_counts(const KParams& p, unsigned long lon
         atomicAdd(&p.counters->steps, n_steps_done);
         atomicAdd(&p.counters->tests, (unsigned long long)work.tests);
         atomicAdd(&p.counters->cells, (unsigned long long)work.cells);
+        atomicAdd(&p.counters->walks, (unsigned long long)work.walks);
     }
 }
 
@@ -830,7 +843,7 @@ struct pp_ctx {
     size_t cell_cap = 0, item_cap = 0;
     int last_search = 0;
     int64_t n_hits = 0, n_ph_sim = 0;
-    int64_t work_steps = 0, work_tests = 0, work_cells = 0;
+    int64_t work_steps = 0, work_tests = 0, work_cells = 0, work_walks = 0;
     float last_ms = 0.0f;
     int32_t n_launch = 0;
     bool perm_valid = false;
@@ -1501,6 +1514,7 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     ctx->work_steps = (int64_t)hc.steps;
     ctx->work_tests = (int64_t)hc.tests;
     ctx->work_cells = (int64_t)hc.cells;
+    ctx->work_walks = (int64_t)hc.walks;
     ctx->perm_valid = false;
     ctx->sorted_valid = false;
     ctx->last_n_sources = n_sources;
@@ -1619,13 +1633,15 @@ int pp_last_timing(pp_ctx* ctx, float* kernel_ms, int32_t* n_launches) {
 
 void* pp_stream(pp_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* cell_visits) {
+int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* cell_visits,
+                   int64_t* grid_walks) {
     if (!ctx) return fail(PP_ERR_INVALID, "pp_work_counts: NULL ctx");
     if (!ctx->cfg.count_work)
         return fail(PP_ERR_STATE, "pp_work_counts: context created without count_work");
     if (steps) *steps = ctx->work_steps;
     if (sphere_tests) *sphere_tests = ctx->work_tests;
     if (cell_visits) *cell_visits = ctx->work_cells;
+    if (grid_walks) *grid_walks = ctx->work_walks;
     return PP_OK;
 }
 

@@ -44,7 +44,7 @@ struct Clip {
 
 // Work done by one search (instrumented builds only).
 struct Work {
-    uint32_t tests = 0, cells = 0;
+    uint32_t tests = 0, cells = 0, walks = 0;
 };
 
 struct BruteSearch {
@@ -186,6 +186,16 @@ struct GridSearch {
         c.t_in = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         c.t_out = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), step));
         return c;
+    }
+    // Whether the bounding box of the segment a-b meets the grid box: a
+    // superset of the segments that clip() keeps (the grid box holds every
+    // target's box with a margin of at least 2 mm, far beyond the rounding of
+    // the end point b), by compares alone, combined without branches
+    __host__ __device__ __forceinline__ bool touches(f3 a, f3 b) const {
+        const bool x = ((a.x >= g.ox) | (b.x >= g.ox)) & ((a.x <= g.ex) | (b.x <= g.ex));
+        const bool y = ((a.y >= g.oy) | (b.y >= g.oy)) & ((a.y <= g.ey) | (b.y <= g.ey));
+        const bool z = ((a.z >= g.oz) | (b.z >= g.oz)) & ((a.z <= g.ez) | (b.z <= g.ez));
+        return x & y & z;
     }
     template <bool COUNT, bool COLS>
     __host__ __device__ __forceinline__ int walk(f3 pos, f3 dir, float step, float& dt,

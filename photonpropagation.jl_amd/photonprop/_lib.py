@@ -164,7 +164,8 @@ SIGNATURES = [
     ("pp_stream", c_void_p, [c_void_p]),
     ("pp_search_info", c_int, [c_void_p, POINTER(c_int32), POINTER(c_float), POINTER(c_int32),
                                POINTER(c_int32)]),
-    ("pp_work_counts", c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
+    ("pp_work_counts", c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),
+                               POINTER(c_int64)]),
     ("pp_pom_pmt_directions", c_int, [POINTER(pp_pom_model), c_void_p]),
     ("pp_pom_pmt_hits", c_int, [c_int, POINTER(pp_pom_model), c_void_p, c_void_p, c_void_p,
                                 c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_uint64,
