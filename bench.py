@@ -95,6 +95,20 @@ def parse():
                     help="skip the PMT-detection leg (pp_pom_pmt_hits on 2^24 hits)")
     ap.add_argument("--pmt-kernels", action="store_true",
                     help="only run the PMT-detection leg (for kernel tracing), then exit")
+    ap.add_argument("--sur-config", action="store_true",
+                    help="only the surrogate's training configuration at its stated size "
+                         "(2e6 samples of 1e7-photon labels, 1-300 m, batch 4096): labels, "
+                         "training and the MC judge, one JSON record, then exit")
+    ap.add_argument("--sur-config-first", type=int, default=0,
+                    help="with --sur-config-save: first sample of the slice to label")
+    ap.add_argument("--sur-config-count", type=int, default=0,
+                    help="with --sur-config-save: samples in the slice (0 = to the end)")
+    ap.add_argument("--sur-config-save", default="",
+                    help="label only the slice and save its labels to this .npz, then exit")
+    ap.add_argument("--sur-config-load", default="",
+                    help="directory of .npz label slices that together cover every sample: "
+                         "train and judge on them instead of labelling")
+    ap.add_argument("--sur-config-steps", type=int, default=48_000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy twice (reference kernel of known "
                          "size for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -500,8 +514,79 @@ def measured_traffic(workload):
     return None, None
 
 
+def surrogate_config(args):
+    """The surrogate's training configuration at its stated size (see
+    photonprop.surrogate.CONFIG_*): MC labels of CONFIG_SAMPLES cascades, each
+    from CONFIG_PHOTONS photons (or the label slices of earlier runs, which
+    are the same labels: same geometry, batches and random streams), Adam
+    training at batch CONFIG_BATCH, and the full-MC judge on held-out events
+    in the 1000-module array with the network's domain 5-300 m."""
+    import glob
+
+    import numpy as np
+
+    import photonprop as P
+    from photonprop import surrogate as S
+    st = S.default_setup()
+    n = S.CONFIG_SAMPLES
+    photons = S.constant_photons(S.CONFIG_PHOTONS)
+    t0 = time.perf_counter()
+    _, dev = P.make_hit_buffers(capacity=1 << 26)
+    with dev:
+        if args.sur_config_save:
+            count = args.sur_config_count or n - args.sur_config_first
+            c, lab, gs = S.generate_training_data(
+                st, n, seed=S.CONFIG_SEED, batch_sources=2048, photons_fn=photons, dev=dev,
+                r_range=S.CONFIG_R_RANGE, first=args.sur_config_first, count=count,
+                log=lambda m: print(m, flush=True))
+            a, b = args.sur_config_first, args.sur_config_first + count
+            S.save_label_slice(args.sur_config_save, a, lab[a:b], gs)
+            print(json.dumps({"slice": [a, b], "label_photons": gs["photons"],
+                              "label_kernel_ms": gs["kernel_ms"], "seconds": gs["seconds"],
+                              "bytes": os.path.getsize(args.sur_config_save)}), flush=True)
+            return
+        if args.sur_config_load:
+            # geometry and exposures from the seed; labels from the slices
+            c, lab, gs = S.generate_training_data(
+                st, n, seed=S.CONFIG_SEED, batch_sources=2048, photons_fn=photons, dev=dev,
+                r_range=S.CONFIG_R_RANGE, first=n, count=0, log=None)
+            covered, gs = S.load_label_slices(
+                sorted(glob.glob(os.path.join(args.sur_config_load, "*.npz"))), lab)
+            if not covered.all():
+                raise SystemExit(f"label slices cover {covered.sum()} of {n} samples")
+        else:
+            c, lab, gs = S.generate_training_data(
+                st, n, seed=S.CONFIG_SEED, batch_sources=2048, photons_fn=photons, dev=dev,
+                r_range=S.CONFIG_R_RANGE, log=lambda m: print(m, flush=True))
+        print(f"labels ready: {gs['photons']:.4g} photons, {time.perf_counter() - t0:.0f} s",
+              flush=True)
+        mlp, info = S.train_surrogate(st, c, lab, steps=args.sur_config_steps,
+                                      batch=S.CONFIG_BATCH, lr=2e-3, lr_stages=5,
+                                      log=lambda m: print(m, flush=True))
+        with mlp:
+            model = S.SurrogateModel(mlp, st, st.domain(max_dist=S.CONFIG_R_RANGE[1]))
+            targets = P.make_detector_grid(10, 10, 50.0, 10, 10.0, radius=0.3)
+            mxyz = np.array([t.shape.position for t in targets], np.float32)
+            held_out = S.array_events(np.random.default_rng(99), args.sur_events, mxyz, setup=st)
+            res, mc, _ = S.judge_events(model, held_out, targets, max_photons=1e9, dev=dev)
+    rec = {"configuration": "surrogate training data and training at the stated size",
+           "samples": n, "photons_per_label": S.CONFIG_PHOTONS,
+           "distance_m": list(S.CONFIG_R_RANGE), "energy_gev": [100.0, 1e6],
+           "batch": S.CONFIG_BATCH, "label_photons": gs["photons"],
+           "label_kernel_ms": gs["kernel_ms"],
+           "label_photons_per_s": gs["photons"] / max(gs["kernel_ms"] * 1e-3, 1e-9),
+           "label_slices": gs.get("slices"),
+           "train": {k: info[k] for k in ("steps", "batch", "steps_per_s", "samples",
+                                          "val_loss", "train_seconds")},
+           "judge": res, "seconds": time.perf_counter() - t0}
+    print(json.dumps(rec, default=float), flush=True)
+
+
 def main():
     args = parse()
+    if args.sur_config:
+        surrogate_config(args)
+        return
     if args.calib_copy:
         calib_copy()
         return

@@ -55,7 +55,7 @@ struct Counters {
 };
 
 struct KParams {
-    pp_source src;
+    const pp_source* sources;  // device copy of the source list (one launch per source)
     pp::Medium med;
     uint32_t k0, k1, src_ix;
     int32_t n_steps;
@@ -519,6 +519,22 @@ __device__ __forceinline__ void flush_counts(const KParams& p, unsigned long lon
 #define PP_PROPAGATE_ATTRS \
     __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PP_WAVES_PER_EU)))
 
+// The source record at sp, read through the constant address space (scalar
+// loads into SGPRs), and the source index, both opaque to the optimiser: a
+// caller that reads them in every iteration of a photon loop gets what the
+// emission derives from the source computed in that iteration, not hoisted
+// out of the loop into registers that spill to scratch.
+__device__ __forceinline__ pp_source source_record(const pp_source* sp, uint32_t& si) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using ConstSource = const __attribute__((address_space(4))) pp_source;
+    ConstSource* c = (ConstSource*)sp;
+    __asm__ volatile("" : "+s"(c), "+s"(si));
+    return *c;
+#else
+    return *sp;
+#endif
+}
+
 template <int PHASE, int SEARCH, bool COUNT = false>
 __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
     const Lds l = stage_lds<SEARCH>(p);
@@ -534,12 +550,14 @@ __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
         bool overflow = false;
         for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
              base < p.n_photons; base += stride) {
+            uint32_t si = p.src_ix;
+            const pp_source src = source_record(p.sources + p.src_ix, si);
             const uint64_t pid = base + __lane_id();
             const bool active = pid < p.n_photons && !overflow;
             HitRec h;
-            const bool hit = transport_group<COUNT>(p, p.src, p.src_ix, pid, active, l.spec, search,
+            const bool hit = transport_group<COUNT>(p, src, si, pid, active, l.spec, search,
                                                     phase, q, work, n_steps_done, h);
-            if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
+            if (!store_hits(hit, h, si, p.hits, p.counters)) {
                 atomicOr(&p.counters->err, 1u);
                 overflow = true;
             } else if (active) {
@@ -550,10 +568,12 @@ __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
     } else {
         for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < p.n_photons;
              pid += stride) {
+            uint32_t si = p.src_ix;
+            const pp_source src = source_record(p.sources + p.src_ix, si);
             HitRec h;
-            const bool hit = transport_photon<COUNT>(p, p.src, p.src_ix, pid, l.spec, search,
+            const bool hit = transport_photon<COUNT>(p, src, si, pid, l.spec, search,
                                                      phase, work, n_steps_done, h);
-            if (!store_hits(hit, h, p.src_ix, p.hits, p.counters)) {
+            if (!store_hits(hit, h, si, p.hits, p.counters)) {
                 atomicOr(&p.counters->err, 1u);
                 break;
             }
@@ -604,12 +624,17 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
             if (b.chunk_prefix[mid] <= ck) lo = mid;
             else hi = mid - 1;
         }
-        const int si = lo;
-        const pp_source src = b.sources[si];
-        const uint64_t n_src = b.n_photons[si];
-        const uint64_t first = (ck - b.chunk_prefix[si]) * b.chunk;
+        const int si0 = lo;
+        const uint64_t n_src = b.n_photons[si0];
+        const uint64_t first = (ck - b.chunk_prefix[si0]) * b.chunk;
         const uint64_t end = first + b.chunk < n_src ? first + b.chunk : n_src;
         for (uint64_t base = first; base < end; base += 64) {
+            // the source index and record, opaque to the optimiser in every
+            // iteration: what the emission derives from them is computed here
+            // (one scalar reload per 64 photons) instead of being hoisted out
+            // of the loop into registers that then spill to scratch
+            uint32_t si = (uint32_t)si0;
+            const pp_source src = source_record(b.sources + si0, si);
             const uint64_t pid = base + lane;
             if constexpr (kDeferredWalk<SEARCH>) {
                 const bool active = pid < end && !overflow;
@@ -977,6 +1002,22 @@ int max_resident_blocks(pp_ctx* c, const void* fn, int block, size_t lds, int64_
     return PP_OK;
 }
 
+// The context's source table (device) and its pinned staging, grown to at
+// least `bytes`; waits for the stream first, since earlier work may read them.
+int ensure_batch_table(pp_ctx* c, size_t bytes) {
+    PP_HIP(hipStreamSynchronize(c->stream));
+    if (bytes <= c->batch_cap) return PP_OK;
+    if (c->d_batch) PP_HIP(hipFree(c->d_batch));
+    if (c->h_batch) PP_HIP(hipHostFree(c->h_batch));
+    c->d_batch = c->h_batch = nullptr;
+    c->batch_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+    PP_HIP(hipMalloc(&c->d_batch, cap));
+    PP_HIP(hipHostMalloc(&c->h_batch, cap, hipHostMallocDefault));
+    c->batch_cap = cap;
+    return PP_OK;
+}
+
 // One launch per source, as src/photon_prop_cuda.jl:925-944.
 int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
                       const std::vector<uint64_t>& nph, int skind, int phase, bool count, size_t lds) {
@@ -995,11 +1036,16 @@ int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
     const int block = c->cfg.block_size;
     int64_t max_blocks = 0;
     if (int rc = max_resident_blocks(c, (const void*)kern, block, lds, max_blocks)) return rc;
+    // the source list on the device (the kernels read their source from it)
+    const size_t bytes = nph.size() * sizeof(pp_source);
+    if (int rc = ensure_batch_table(c, bytes)) return rc;
+    std::memcpy(c->h_batch, sources, bytes);
+    PP_HIP(hipMemcpyAsync(c->d_batch, c->h_batch, bytes, hipMemcpyHostToDevice, c->stream));
+    kp.sources = (const pp_source*)c->d_batch;
     PP_HIP(hipMemsetAsync(c->d_counters, 0, sizeof(Counters), c->stream));
     PP_HIP(hipEventRecord(c->ev0, c->stream));
     for (size_t i = 0; i < nph.size(); ++i) {
         if (nph[i] == 0) continue;
-        kp.src = sources[i];
         kp.src_ix = (uint32_t)i;
         kp.n_photons = nph[i];
         const int64_t need = (int64_t)((nph[i] + block - 1) / block);
@@ -1051,17 +1097,7 @@ int launch_batch(pp_ctx* c, const KParams& kp, const pp_source* sources,
     const size_t off_n = ((n * sizeof(pp_source)) + 255) & ~(size_t)255;
     const size_t off_p = off_n + ((n * 8 + 255) & ~(size_t)255);
     const size_t bytes = off_p + (n + 1) * 8;
-    if (bytes > c->batch_cap) {
-        PP_HIP(hipStreamSynchronize(c->stream));
-        if (c->d_batch) PP_HIP(hipFree(c->d_batch));
-        if (c->h_batch) PP_HIP(hipHostFree(c->h_batch));
-        c->d_batch = c->h_batch = nullptr;
-        c->batch_cap = 0;
-        const size_t cap = std::max<size_t>(bytes, 64 * 1024);
-        PP_HIP(hipMalloc(&c->d_batch, cap));
-        PP_HIP(hipHostMalloc(&c->h_batch, cap, hipHostMallocDefault));
-        c->batch_cap = cap;
-    }
+    if (int rc = ensure_batch_table(c, bytes)) return rc;
     char* h = (char*)c->h_batch;
     std::memcpy(h, sources, n * sizeof(pp_source));
     uint64_t* hn = (uint64_t*)(h + off_n);

@@ -233,15 +233,71 @@ def photons_for_distance(r, lo=2e4, hi=4e6, scale=2e4):
     return np.clip(scale * np.asarray(r) ** 2, lo, hi).astype(np.int64)
 
 
+# The surrogate's training configuration at its stated size: 2e6 cascades
+# around one module, distance log-uniform 1-300 m, energy log-uniform
+# 100 GeV - 1 PeV, labels from 1e7-photon runs, Adam at batch 4096.
+CONFIG_SAMPLES = 2_000_000
+CONFIG_PHOTONS = 10_000_000
+CONFIG_R_RANGE = (1.0, 300.0)
+CONFIG_BATCH = 4096
+CONFIG_SEED = 2026
+
+
+def save_label_slice(path, first, labels, stats):
+    """Labels of samples [first, first + len(labels)) (rows as returned by
+    generate_training_data) to a compressed .npz: the weighted hit total in
+    float32 and each time bin as a float16 fraction of it (relative error at
+    most 2^-11 of the bin, far below the Monte Carlo's own), with the run's
+    photon count and times."""
+    tot = labels[:, 1].astype(np.float32)
+    frac = np.where(tot[:, None] > 0, labels[:, 2:] / np.maximum(tot, 1e-30)[:, None], 0.0)
+    np.savez_compressed(path, first=first, count=len(labels), total=tot,
+                        frac=frac.astype(np.float16), photons=stats["photons"],
+                        kernel_ms=stats["kernel_ms"], seconds=stats["seconds"])
+
+
+def load_label_slices(paths, labels):
+    """Fill labels[:, 1:] from save_label_slice files; returns (rows covered
+    (bool mask), summed photons, kernel ms and seconds, [first, end) list)."""
+    covered = np.zeros(len(labels), bool)
+    out = dict(photons=0, kernel_ms=0.0, seconds=0.0, slices=[])
+    for f in paths:
+        z = np.load(f)
+        a, k = int(z["first"]), int(z["count"])
+        labels[a:a + k, 1] = z["total"]
+        labels[a:a + k, 2:] = z["frac"].astype(np.float32) * z["total"][:, None]
+        covered[a:a + k] = True
+        out["photons"] += int(z["photons"])
+        out["kernel_ms"] += float(z["kernel_ms"])
+        out["seconds"] += float(z["seconds"])
+        out["slices"].append([a, a + k])
+    return covered, out
+
+
+def constant_photons(n):
+    """photons_fn of generate_training_data: n photons for every sample."""
+    return lambda r: np.full(np.shape(r), n, np.int64)
+
+
 def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sources=2048,
                            photons_fn=photons_for_distance, dev=None, log=print,
-                           r_range=(1.0, 800.0), device_labels=True):
+                           r_range=(1.0, 800.0), device_labels=True, first=0, count=None):
     """Labels from the Monte Carlo: every sample is a cascade (vertex, axis,
     energy) around one module at the origin; batches of samples run as the
     sources of one persistent propagation launch. With device_labels the
     hits are reduced to labels on the device (pp_sur_hit_labels: only the
     labels come back); otherwise the hits are copied and reduced here.
+    first/count select a slice of the n_samples (first a multiple of
+    batch_sources; count too unless the slice ends at n_samples): the slice's
+    labels are those of the whole run (same geometry, same batches, same
+    random streams), the other rows stay zero apart from the exposure.
     Returns (cascades (n, 8), labels (n, 66), stats)."""
+    if count is None:
+        count = n_samples - first
+    end = first + count
+    if not (0 <= first <= end <= n_samples) or count and (
+            first % batch_sources or (count % batch_sources and end != n_samples)):
+        raise ValueError("generate_training_data: the slice must follow the batch boundaries")
     lib = _lib.load_library()
     rng = np.random.default_rng(seed)
     pos, d, e = sample_geometry(rng, n_samples, r_range=r_range)
@@ -295,8 +351,8 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
     try:
         if device_labels:
             stage(dev, [target], medium, setup.device_spectrum)
-        for b0 in range(0, n_samples, batch_sources):
-            b1 = min(n_samples, b0 + batch_sources)
+        for b0 in range(first, end, batch_sources):
+            b1 = min(end, b0 + batch_sources)
             if device_labels:
                 device_batch(b0, b1, seed * 1000003 + b0)
             else:
@@ -318,7 +374,7 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
                                    minlength=(b1 - b0) * BINS)
                 labels[b0:b1, 2:] += flat.reshape(b1 - b0, BINS).astype(np.float32)
             if log and (b0 // batch_sources) % 10 == 0:
-                log(f"  labels {b1}/{n_samples}: {photons:.3g} photons, "
+                log(f"  labels {b1 - first}/{count}: {photons:.3g} photons, "
                     f"{time.perf_counter() - t0:.1f} s")
     finally:
         if own:
@@ -326,7 +382,7 @@ def generate_training_data(setup: SurrogateSetup, n_samples, seed=1, batch_sourc
     cascades = np.concatenate([pos, d, e[:, None].astype(np.float32),
                                logy[:, None].astype(np.float32)], axis=1)
     stats = dict(photons=photons, seconds=time.perf_counter() - t0, kernel_ms=kernel_ms,
-                 samples=n_samples)
+                 samples=n_samples, first=first, count=count)
     return cascades.astype(np.float32), labels, stats
 
 

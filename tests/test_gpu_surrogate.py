@@ -136,6 +136,30 @@ def test_adam_step_matches_oracle(S, O):
         np.testing.assert_allclose(m.get_params(), po2, rtol=5e-6, atol=1e-8)
 
 
+def test_batch_4096_gradient_and_adam_step_match_oracle(S, O):
+    """At the training batch of the surrogate's configuration (4096 rows):
+    the loss and every layer's gradient against the oracle, then one Adam
+    step on the kernel's gradient against the oracle's Adam."""
+    rng = np.random.default_rng(7)
+    with S.MLP(max_batch=4096, seed=11) as m:
+        p = m.get_params()
+        x, lab = feats(rng, 4096), labels_for(rng, 4096)
+        loss, g = m.loss_grad(x, lab)
+        lo, go = O.mlp_loss_grad(m.dims, p, x, lab, quant=True)
+        print(f"batch 4096: loss gpu {loss:.6g} oracle {lo:.6g}")
+        assert loss == pytest.approx(lo, rel=1e-3)
+        cos = float(g.astype(np.float64) @ go / np.linalg.norm(g) / np.linalg.norm(go))
+        assert cos > 0.999, cos
+        for name, sl in layer_slices(m.dims):
+            rel = np.linalg.norm(g[sl] - go[sl]) / max(np.linalg.norm(go[sl]), 1e-30)
+            print(f"{name}: rel L2 {rel:.2e}")
+            assert rel < 3e-2, name
+        m.adam_step(lr=2e-3)
+        z = np.zeros_like(p)
+        po, _, _ = O.adam(p, g, z, z, 0, 2e-3)
+        np.testing.assert_allclose(m.get_params(), po, rtol=2e-6, atol=1e-9)
+
+
 def _philox_indices(O, seed, step, rows, n):
     ctr = np.stack([np.arange(rows), np.full(rows, step), np.full(rows, 0x5EEDBA7C),
                     np.zeros(rows)], axis=1).astype(np.uint32)

@@ -93,3 +93,40 @@ def test_base_weight_division_semantics(P):
 def test_module_index_outside_the_target_list(P):
     with pytest.raises(IndexError):
         P.processing.assemble_hit_table(hits_dict(1, [450.0], [1.0], [4]), 10, make_setup(P))
+
+
+def test_training_data_slices_follow_batch_boundaries(P):
+    """generate_training_data labels a slice of the samples only along the
+    batch boundaries of the whole run (so that a slice's random streams and
+    labels are those of the whole run); other slices are refused before any
+    device work."""
+    from photonprop import surrogate as S
+    st = S.default_setup()
+    for first, count in ((1, 2048), (0, 1000), (2048, 9000), (0, 20000)):
+        with pytest.raises(ValueError):
+            S.generate_training_data(st, 10_000, batch_sources=2048, first=first, count=count,
+                                     dev=None, log=None)
+
+
+def test_label_slices_round_trip(P, tmp_path):
+    """save_label_slice / load_label_slices: totals exact, time bins within
+    2^-11 of their value (float16 fractions of the total) or 1e-7 of the
+    total for the tiniest fractions, empty bins empty, rows of two slices
+    placed at their own sample indices."""
+    from photonprop import surrogate as S
+    rng = np.random.default_rng(4)
+    lab = np.zeros((6000, 66), np.float32)
+    lab[:, 0] = 1.0
+    lab[:, 2:] = rng.gamma(0.3, 50.0, (6000, 64)) * (rng.random((6000, 64)) < 0.4)
+    lab[:, 2:] *= np.exp(-rng.uniform(0, 12, (6000, 1)))  # absorption weights down to 6e-6
+    lab[:, 1] = lab[:, 2:].sum(1) + rng.uniform(0, 5, 6000)  # hits outside the bins
+    stats = dict(photons=123, kernel_ms=1.5, seconds=2.5)
+    S.save_label_slice(tmp_path / "a.npz", 0, lab[:4096], stats)
+    S.save_label_slice(tmp_path / "b.npz", 4096, lab[4096:], stats)
+    out = np.zeros_like(lab)
+    cov, st = S.load_label_slices([tmp_path / "b.npz", tmp_path / "a.npz"], out)
+    assert cov.all() and st["photons"] == 246 and sorted(st["slices"]) == [[0, 4096], [4096, 6000]]
+    assert np.array_equal(out[:, 1], lab[:, 1])
+    err = np.abs(out[:, 2:] - lab[:, 2:])
+    assert np.all(err <= np.maximum(lab[:, 2:] * 2.0**-11, lab[:, 1:2] * 1e-7))
+    assert np.all(out[:, 2:][lab[:, 2:] == 0] == 0)
