@@ -295,12 +295,17 @@ __host__ __device__ __forceinline__ bool plane_hit(uint32_t kind, float4 t, floa
 struct Medium {
     int kind;  // pp_medium_kind
     Phase phase;
-    // SIMPLE
-    float sca_len, c_grp, n_phase;
+    // Values derived from launch constants are stored here, computed on the
+    // host: the kernels would otherwise hoist each conversion or product into
+    // a register that spills, and reload it from scratch for every photon.
+    // SIMPLE (inv_c_grp = 1 / c_grp)
+    float sca_len, c_grp, n_phase, inv_c_grp;
     // CASCADIA: Kopelevich b = k1 x^-4.3 + k2 x^-1.7 + k3 x^-0.3, x = wl/550
-    float kop1, kop2, kop3, sca_scale;
+    // (FP64 copies of the Float32 coefficients)
+    double kop1, kop2, kop3, sca_scale;
     // Quan-Fry: n = a0 + a1/wl + a2/wl^2 + a3/wl^3 ; n_g = a0 + 2a1/wl + 3a2/wl^2 + 4a3/wl^3
-    float qf0, qf1, qf2, qf3;
+    // (qg1..qg3 = 2 a1, 3 a2, 4 a3 in Float32)
+    float qf0, qf1, qf2, qf3, qg1, qg2, qg3;
     // ICE: sca_len = ice_pref (wl/400)^ice_alpha, ice_pref = sca_scale (1-g) / b_dust_400
     float ice_pref, ice_alpha;
 };
@@ -314,7 +319,7 @@ __device__ __forceinline__ MediumValues medium_values(const Medium& m, float wl)
     float n;
     if (m.kind == PP_MEDIUM_SIMPLE) {
         v.sca_len = m.sca_len;
-        v.inv_c_grp = 1.0f / m.c_grp;
+        v.inv_c_grp = m.inv_c_grp;
         n = m.n_phase;
     } else if (m.kind == PP_MEDIUM_ICE) {
         // src/ice_properties.jl:105-126 (indices), :164-168 (scattering)
@@ -336,14 +341,12 @@ __device__ __forceinline__ MediumValues medium_values(const Medium& m, float wl)
         const double q2 = q * q;
         const float tf = __builtin_amdgcn_exp2f(0.3f * __builtin_amdgcn_logf((float)q));
         const double t = (double)tf;
-        const double b = fma((double)m.kop1, q2 * q2 * t,
-                             fma((double)m.kop2, q2 * rcp64(t), (double)m.kop3 * t));
-        v.sca_len = (float)((double)m.sca_scale * rcp64(b));
+        const double b = fma(m.kop1, q2 * q2 * t, fma(m.kop2, q2 * rcp64(t), m.kop3 * t));
+        v.sca_len = (float)(m.sca_scale * rcp64(b));
         const float il = rcp_nr(wl);
         // Horner with FMAs
         n = fmaf(il, fmaf(il, fmaf(il, m.qf3, m.qf2), m.qf1), m.qf0);
-        const float ng =
-            fmaf(il, fmaf(il, fmaf(il, 4.0f * m.qf3, 3.0f * m.qf2), 2.0f * m.qf1), m.qf0);
+        const float ng = fmaf(il, fmaf(il, fmaf(il, m.qg3, m.qg2), m.qg1), m.qf0);
         v.inv_c_grp = ng * (1.0f / kCVacMNs);
     }
     v.cos_c = rcp_nr(n);  // cherenkov_angle: acos(1/n), beta = 1
@@ -431,13 +434,16 @@ __device__ inline float gamma_sample(float shape, float scale, const Stream& s, 
 // ---------------------------------------------------------------------------
 // Spectrum: inverse-CDF knots, linear interpolation
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float spectrum_sample(const float* tab, int n, int mapping, float mono,
-                                                 float u) {
+// n_f = float(n), last_f = float(n - 1): passed in, computed once on the host
+// (a conversion of a launch constant would otherwise be hoisted into a
+// register that spills)
+__device__ __forceinline__ float spectrum_sample(const float* tab, int n, float n_f, float last_f,
+                                                 int mapping, float mono, float u) {
     if (n == 0) return mono;
     // separately rounded products and sums (this header turns contraction
     // off): the knot position and a + f (b - a) as the reference evaluates them
-    float x = mapping == PP_SPEC_TEXTURE ? u * (float)n - 0.5f : u * (float)(n - 1);
-    x = fminf(fmaxf(x, 0.0f), (float)(n - 1));
+    float x = mapping == PP_SPEC_TEXTURE ? u * n_f - 0.5f : u * last_f;
+    x = fminf(fmaxf(x, 0.0f), last_f);
     const int i = min((int)x, n - 2);
     const float f = x - (float)i;
     const float a = tab[i], b = tab[i + 1];

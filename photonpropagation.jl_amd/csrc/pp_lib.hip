@@ -69,6 +69,7 @@ struct KParams {
     int32_t spec_n;
     int32_t spec_map;
     float spec_mono;
+    float spec_nf, spec_lastf;  // float(spec_n), float(spec_n - 1)
     const float* spec;
     HitCols hits;
     Counters* counters;
@@ -111,7 +112,8 @@ __device__ __forceinline__ PhotonInit init_photon(const KParams& p, const pp_sou
                                                   const float* spec, const pp::Stream& s) {
     const uint4 r = s.block(PP_CTR_INIT);
     PhotonInit st;
-    st.wl = pp::spectrum_sample(spec, p.spec_n, p.spec_map, p.spec_mono, pp::u01(r.x));
+    st.wl = pp::spectrum_sample(spec, p.spec_n, p.spec_nf, p.spec_lastf, p.spec_map, p.spec_mono,
+                                pp::u01(r.x));
     st.mv = pp::medium_values(p.med, st.wl);
     st.pos = pp::make3(src.position[0], src.position[1], src.position[2]);
     st.time = src.time;
@@ -387,6 +389,11 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
                                                 HitRec& h) {
     const uint32_t lane = __lane_id();
     pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), src_ix};
+    // the wave's photon ids are consecutive: pid = wave_first + lane
+    const uint64_t pid0 = pid - lane;
+    const uint64_t wave_first =
+        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pid0 >> 32)) << 32) |
+        __builtin_amdgcn_readfirstlane((uint32_t)pid0);
     f3 pos = pp::make3(0.0f, 0.0f, 0.0f), dir = pos;
     float ln2_sca = 0.0f;
     if (active) {
@@ -396,7 +403,11 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         dir = st.dir;
         ln2_sca = st.mv.sca_len * pp::kLn2;
     }
-    q.key[lane] = ~0ull;
+    // no hit yet: all ones, materialised here (a 64-bit constant kept in a
+    // register pair across the photon loop was spilled and reloaded per photon)
+    uint32_t ones;
+    __asm__ volatile("v_mov_b32 %0, -1" : "=v"(ones));
+    *reinterpret_cast<uint2*>(q.key + lane) = make_uint2(ones, ones);
     // going: still stepping (active, and no hit found so far; steps after a
     // found hit cannot change the earliest one)
     bool going = active, stop = false;
@@ -490,15 +501,24 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
             rd = pp::update_direction(rd, ct, pp::u01(r.w));
         }
         move_by(rp, rdist, rt, rd, dtk, inv_c);
-        h = HitRec{rp, rd, st.dir, st.wl, rt, rdist, (int32_t)(key & 0xFFFFu) + 1, k, pid};
+        // the photon id recomputed from the wave's first id (scalar) and the
+        // lane: a per-lane copy kept for the rare hit was spilled per photon
+        uint64_t first = wave_first;
+        __asm__ volatile("" : "+s"(first));
+        const uint64_t id = first + lane;
+        h = HitRec{rp, rd, st.dir, st.wl, rt, rdist, (int32_t)(key & 0xFFFFu) + 1, k, id};
     }
     return hit;
 }
 
+// n_done: this lane's completed photons; wave_done: the wave's (the same
+// value in every lane, added once)
 template <bool COUNT>
 __device__ __forceinline__ void flush_counts(const KParams& p, unsigned long long n_done,
+                                             unsigned long long wave_done,
                                              unsigned long long n_steps_done, const pp::Work& work) {
     if (n_done) atomicAdd(&p.counters->n_ph_sim, n_done);
+    if (wave_done && __lane_id() == 0) atomicAdd(&p.counters->n_ph_sim, wave_done);
     if (COUNT) {
         atomicAdd(&p.counters->steps, n_steps_done);
         atomicAdd(&p.counters->tests, (unsigned long long)work.tests);
@@ -518,6 +538,18 @@ __device__ __forceinline__ void flush_counts(const KParams& p, unsigned long lon
 #endif
 #define PP_PROPAGATE_ATTRS \
     __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PP_WAVES_PER_EU)))
+
+// *p read through the constant address space: a scalar load into SGPRs for
+// a wave-uniform address (the tables the kernels read are not written
+// during a launch)
+template <typename T>
+__device__ __forceinline__ T load_constant(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T*)p;
+#else
+    return *p;
+#endif
+}
 
 // The source record at sp, read through the constant address space (scalar
 // loads into SGPRs), and the source index, both opaque to the optimiser: a
@@ -542,7 +574,9 @@ __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
     phase.kind = PHASE;
     const auto search = make_search<SEARCH>(p, l);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    unsigned long long n_done = 0, n_steps_done = 0;
+    // photons completed: per lane in the one-photon loop, per wave (a scalar,
+    // never spilled) in the wave loop
+    unsigned long long n_done = 0, wave_done = 0, n_steps_done = 0;
     pp::Work work;
     if constexpr (kDeferredWalk<SEARCH>) {
         // whole waves of photons, the loop condition uniform across the wave
@@ -557,12 +591,12 @@ __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
             HitRec h;
             const bool hit = transport_group<COUNT>(p, src, si, pid, active, l.spec, search,
                                                     phase, q, work, n_steps_done, h);
-            if (!store_hits(hit, h, si, p.hits, p.counters)) {
+            const bool stored = store_hits(hit, h, si, p.hits, p.counters);
+            if (!stored) {
                 atomicOr(&p.counters->err, 1u);
                 overflow = true;
-            } else if (active) {
-                ++n_done;
             }
+            wave_done += (unsigned long long)__popcll(__ballot(stored && active));
             if (__ballot(overflow)) break;
         }
     } else {
@@ -580,7 +614,7 @@ __global__ void PP_PROPAGATE_ATTRS propagate_kernel(KParams p) {
             ++n_done;
         }
     }
-    flush_counts<COUNT>(p, n_done, n_steps_done, work);
+    flush_counts<COUNT>(p, n_done, wave_done, n_steps_done, work);
 }
 
 // Batch of sources in one persistent launch. Work items are chunks of
@@ -604,11 +638,11 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
     pp::Phase phase = p.med.phase;
     phase.kind = PHASE;
     const auto search = make_search<SEARCH>(p, l);
-    const uint64_t total = b.chunk_prefix[b.n_sources];
+    const uint64_t total = load_constant(b.chunk_prefix + b.n_sources);
     const uint32_t lane = __lane_id();
     WaveQueue q{};
     if constexpr (kDeferredWalk<SEARCH>) q = wave_queue(p);
-    unsigned long long n_done = 0, n_steps_done = 0;
+    unsigned long long n_done = 0, wave_done = 0, n_steps_done = 0;
     pp::Work work;
     bool overflow = false;
     for (;;) {
@@ -621,12 +655,12 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
         int lo = 0, hi = b.n_sources - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (b.chunk_prefix[mid] <= ck) lo = mid;
+            if (load_constant(b.chunk_prefix + mid) <= ck) lo = mid;
             else hi = mid - 1;
         }
         const int si0 = lo;
-        const uint64_t n_src = b.n_photons[si0];
-        const uint64_t first = (ck - b.chunk_prefix[si0]) * b.chunk;
+        const uint64_t n_src = load_constant(b.n_photons + si0);
+        const uint64_t first = (ck - load_constant(b.chunk_prefix + si0)) * b.chunk;
         const uint64_t end = first + b.chunk < n_src ? first + b.chunk : n_src;
         for (uint64_t base = first; base < end; base += 64) {
             // the source index and record, opaque to the optimiser in every
@@ -641,12 +675,12 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
                 HitRec h;
                 const bool hit = transport_group<COUNT>(p, src, (uint32_t)si, pid, active, l.spec,
                                                         search, phase, q, work, n_steps_done, h);
-                if (!store_hits(hit, h, (uint32_t)si, p.hits, p.counters)) {
+                const bool stored = store_hits(hit, h, (uint32_t)si, p.hits, p.counters);
+                if (!stored) {
                     atomicOr(&p.counters->err, 1u);
                     overflow = true;
-                } else if (active) {
-                    ++n_done;
                 }
+                wave_done += (unsigned long long)__popcll(__ballot(stored && active));
             } else {
                 if (pid >= end || overflow) continue;
                 HitRec h;
@@ -662,7 +696,7 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
         }
         if (__ballot(overflow)) break;  // the call fails; stop taking chunks
     }
-    flush_counts<COUNT>(p, n_done, n_steps_done, work);
+    flush_counts<COUNT>(p, n_done, wave_done, n_steps_done, work);
 }
 
 // Zeroes the counters in a captured graph (a kernel node, not a memset node).
@@ -683,6 +717,7 @@ struct EvalParams {
     const float* spec;
     int spec_n, spec_map;
     float spec_mono;
+    float spec_nf, spec_lastf;  // float(spec_n), float(spec_n - 1)
 };
 
 __device__ __forceinline__ uint32_t bits(float x) { return __float_as_uint(x); }
@@ -758,7 +793,8 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         break;
     }
     case PP_FN_SPECTRUM:
-        y[0] = pp::spectrum_sample(e.spec, e.spec_n, e.spec_map, e.spec_mono, x[0]);
+        y[0] = pp::spectrum_sample(e.spec, e.spec_n, e.spec_nf, e.spec_lastf, e.spec_map,
+                                   e.spec_mono, x[0]);
         break;
     case PP_FN_SL_COS: {
         pp::Phase ph = pp::make_phase(PP_PHASE_MIXED_HG_SL, x[0], 1.0f, 0.0f);
@@ -810,11 +846,12 @@ pp::Medium lower_medium(const pp_medium& m) {
     d.phase = pp::make_phase(m.phase_kind, m.hg_g, m.es_b, m.hg_fraction);
     d.sca_len = m.sca_len;
     d.c_grp = m.group_velocity;
+    d.inv_c_grp = 1.0f / m.group_velocity;
     d.n_phase = m.phase_refractive_index;
-    d.sca_scale = m.sca_scale;
-    d.kop1 = 0.0017f;
-    d.kop2 = (float)(1.34 * (double)m.vol_conc_small);
-    d.kop3 = (float)(0.312 * (double)m.vol_conc_large);
+    d.sca_scale = (double)m.sca_scale;
+    d.kop1 = (double)0.0017f;
+    d.kop2 = (double)(float)(1.34 * (double)m.vol_conc_small);
+    d.kop3 = (double)(float)(0.312 * (double)m.vol_conc_large);
     // Quan & Fry (1995) coefficients; linear pressure term 1.45e-5 / atm
     const double S = m.salinity, T = m.temperature, P = m.pressure;
     d.qf0 = (float)(1.31405 + (1.779e-4 - 1.05e-6 * T + 1.6e-8 * T * T) * S - 2.02e-6 * T * T +
@@ -822,6 +859,9 @@ pp::Medium lower_medium(const pp_medium& m) {
     d.qf1 = (float)(15.868 + 0.01155 * S - 0.00423 * T);
     d.qf2 = (float)(-4382.0);
     d.qf3 = (float)(1.1455e6);
+    d.qg1 = 2.0f * d.qf1;
+    d.qg2 = 3.0f * d.qf2;
+    d.qg3 = 4.0f * d.qf3;
     if (m.kind == PP_MEDIUM_ICE) {
         d.ice_pref = (float)((double)m.sca_scale * (1.0 - (double)m.hg_g) / (double)m.ice_b_dust_400);
         d.ice_alpha = m.ice_alpha_sca_dust;
@@ -1519,6 +1559,8 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     kp.spec_n = ctx->spec_n;
     kp.spec_map = ctx->spec_map;
     kp.spec_mono = ctx->spec_mono;
+    kp.spec_nf = (float)ctx->spec_n;
+    kp.spec_lastf = (float)(ctx->spec_n - 1);
     kp.spec = ctx->d_spec;
     kp.hits = ctx->hits;
     kp.counters = ctx->d_counters;
@@ -2028,6 +2070,8 @@ int pp_eval_fn(pp_ctx* ctx, int fn, const float* in, int in_stride, float* out, 
         e.spec_n = ctx->spec_n;
         e.spec_map = ctx->spec_map;
         e.spec_mono = ctx->spec_mono;
+        e.spec_nf = (float)ctx->spec_n;
+        e.spec_lastf = (float)(ctx->spec_n - 1);
         const int64_t grid = (n + 255) / 256;
         hipLaunchKernelGGL(eval_fn_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, e);
         if (hipGetLastError() != hipSuccess) { rc = fail(PP_ERR_HIP, "eval launch"); break; }
