@@ -14,11 +14,13 @@
 
 #include "../../include/photonprop_hip.h"
 
-// No contraction in any translation unit that includes this header: each
-// fused multiply-add is written as fmaf and every other product and sum is
-// rounded on its own. A function then gives the same bits wherever it is
-// inlined (the propagation kernels inline the step functions into several
-// schedules whose results must agree bit for bit).
+// No contraction from here on in a translation unit that includes this
+// header: each fused multiply-add is written as fmaf and every other product
+// and sum is rounded on its own. A function then gives the same bits wherever
+// it is inlined (the propagation kernels inline the step functions into
+// several schedules whose results must agree bit for bit). The setting is
+// taken when a function is parsed, so a file that switches contraction back
+// on for its own code after the include leaves these functions as they are.
 #pragma clang fp contract(off)
 
 namespace pp {
