@@ -425,6 +425,8 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *   PP_FN_CASCADIA        in (wl)                out (sca_len, c_grp, n_phase)
  *   PP_FN_TRACK_DIR       in (u1,u2)             out (x,y,z)
  *   PP_FN_PHILOX          in (c0..c3,k0,k1 as u32 bits) out (r0..r3 as u32 bits)
+ *                         through the steps' per-photon form; all ones if it
+ *                         differs from the emission's plain ten rounds
  *   PP_FN_GAMMA           in (k, theta, key_lo, key_hi, pid_lo, pid_hi, src as u32 bits)
  *                         out (draw, accepted normal variate)
  *   PP_FN_SPECTRUM        in (u)                 out (wl)  (uses the context's spectrum)

@@ -38,17 +38,26 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
-__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+
+// R rounds on c with the round keys (k0, k1) of the first of them
+template <int R>
+__device__ __forceinline__ uint4 philox_rounds(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c.x;
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c.z;
+    for (int r = 0; r < R; ++r) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * (uint64_t)c.x;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * (uint64_t)c.z;
         c = make_uint4(xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1,
                        xor3((uint32_t)(p0 >> 32), c.w, k1), (uint32_t)p0);
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
     }
     return c;
+}
+
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+    return philox_rounds<10>(c, k0, k1);
 }
 
 // [0,1) and (0,1] with 24 random bits (exact integer->float conversion)
@@ -64,6 +73,44 @@ struct Stream {
     uint32_t src;
     __device__ __forceinline__ uint4 block(uint32_t word) const {
         return philox(make_uint4(word, pid_lo, pid_hi, src), k0, k1);
+    }
+};
+
+// The step draws of one photon: Stream::block(word), bit for bit, with the
+// word-independent half of rounds 1 and 2 done once per photon. Round 1
+// multiplies c.z (the photon id's high word) and round 2 the x word that
+// product yields, neither of which depends on the word; the word enters
+// through c.x, and while a wave steps together it is wave-uniform, so its
+// products and the round keys stay scalar. Per photon: y1 = lo(M1 pid_hi),
+// and h2, w2 the high and low halves of M0 x1, x1 = hi(M1 pid_hi)^pid_lo^k0.
+struct StepStream {
+    uint32_t k0, k1, src;
+    uint32_t y1, h2, w2;
+    __device__ __forceinline__ explicit StepStream(const Stream& s)
+        : k0(s.k0), k1(s.k1), src(s.src) {
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * (uint64_t)s.pid_hi;
+        const uint32_t x1 = xor3((uint32_t)(p1 >> 32), s.pid_lo, s.k0);
+        const uint64_t q = (uint64_t)kPhiloxM0 * (uint64_t)x1;
+        y1 = (uint32_t)p1;
+        h2 = (uint32_t)(q >> 32);
+        w2 = (uint32_t)q;
+    }
+    __device__ __forceinline__ uint4 block(uint32_t word) const {
+        // round 1: (x1, y1, z1, w1) with z1, w1 from the word
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * (uint64_t)word;
+        const uint32_t z1 = (uint32_t)(p0 >> 32) ^ src ^ k1;
+        const uint32_t w1 = (uint32_t)p0;
+        // round 2, keys + (W0, W1): M1 z1 from the word, M0 x1 per photon
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * (uint64_t)z1;
+        const uint32_t ka = (uint32_t)(p1 >> 32) ^ (k0 + kPhiloxW0);
+        const uint32_t kb = w1 ^ (k1 + kPhiloxW1);
+        const uint32_t x2 = y1 ^ ka, y2 = (uint32_t)p1, z2 = h2 ^ kb;
+        // round 3, keys + 2 (W0, W1): its y word (y2) is still scalar
+        const uint64_t q0 = (uint64_t)kPhiloxM0 * (uint64_t)x2;
+        const uint64_t q1 = (uint64_t)kPhiloxM1 * (uint64_t)z2;
+        const uint4 c = make_uint4((uint32_t)(q1 >> 32) ^ (y2 ^ (k0 + 2u * kPhiloxW0)), (uint32_t)q1,
+                                   xor3((uint32_t)(q0 >> 32), w2, k1 + 2u * kPhiloxW1), (uint32_t)q0);
+        return philox_rounds<7>(c, k0 + 3u * kPhiloxW0, k1 + 3u * kPhiloxW1);
     }
 };
 

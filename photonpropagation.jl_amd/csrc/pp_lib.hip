@@ -305,13 +305,14 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
                                                  HitRec& h) {
     pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), src_ix};
     const PhotonInit st = init_photon(p, src, spec, s);
+    const pp::StepStream ss(s);
     f3 pos = st.pos, dir = st.dir;
     float time = st.time, dist = 0.0f;
     const float ln2_sca = st.mv.sca_len * pp::kLn2, inv_c = st.mv.inv_c_grp;
     bool hit = false;
     int32_t module_ix = 0, n_at = 0;
     for (int32_t step = 1; step <= p.n_steps; ++step) {
-        const uint4 r = s.block((uint32_t)step);
+        const uint4 r = ss.block((uint32_t)step);
         const float step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
         // first target in index order whose near root lies inside the step
         float dt = 0.0f;
@@ -389,6 +390,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
                                                 HitRec& h) {
     const uint32_t lane = __lane_id();
     pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), src_ix};
+    const pp::StepStream ss(s);
     // the wave's photon ids are consecutive: pid = wave_first + lane
     const uint64_t pid0 = pid - lane;
     const uint64_t wave_first =
@@ -451,12 +453,14 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         if (!stepping && count == 0) break;
         const bool move = stepping && going;
         if (stepping) ++step;
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        float step_size = 0.0f;
-        f3 next = pos;
+        // r, step_size and next are set and read only where move holds (no
+        // register is set for the lanes that stay)
+        uint4 r;
+        float step_size;
+        f3 next;
         bool need = false;
         if (move) {
-            r = s.block((uint32_t)step);
+            r = ss.block((uint32_t)step);
             step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
             next = add_scaled(pos, step_size, dir);
             // a walk only for a segment near the grid; the walk clips it exactly
@@ -494,7 +498,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         f3 rp = st.pos, rd = st.dir;
         float rt = st.time, rdist = 0.0f;
         for (int32_t i = 1; i < k; ++i) {
-            const uint4 r = s.block((uint32_t)i);
+            const uint4 r = ss.block((uint32_t)i);
             const float step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
             move_by(rp, rdist, rt, rd, step_size, inv_c);
             const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
@@ -780,8 +784,13 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         break;
     }
     case PP_FN_PHILOX: {
-        uint4 r = pp::philox(make_uint4(bits(x[0]), bits(x[1]), bits(x[2]), bits(x[3])),
-                             bits(x[4]), bits(x[5]));
+        // the step path's StepStream, Stream{k0, k1, c1, c2, c3}.block(c0), and
+        // the plain ten rounds of the emission draws: all ones where they differ
+        const pp::Stream s{bits(x[4]), bits(x[5]), bits(x[1]), bits(x[2]), bits(x[3])};
+        uint4 r = pp::StepStream(s).block(bits(x[0]));
+        const uint4 g = pp::philox(make_uint4(bits(x[0]), bits(x[1]), bits(x[2]), bits(x[3])),
+                                   bits(x[4]), bits(x[5]));
+        if (r.x != g.x || r.y != g.y || r.z != g.z || r.w != g.w) r = make_uint4(~0u, ~0u, ~0u, ~0u);
         y[0] = as_f(r.x); y[1] = as_f(r.y); y[2] = as_f(r.z); y[3] = as_f(r.w);
         break;
     }

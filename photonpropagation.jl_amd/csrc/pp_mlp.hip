@@ -27,8 +27,8 @@
 
 // pp_device.h turns contraction off for the propagation step functions; the
 // network's epilogues (bias, SiLU, Adam, loss) contract products and sums into
-// fused multiply-adds, as hipcc does by default. Their tolerance to the
-// oracle is set by the bf16 operands, not by the rounding of these sums.
+// fused multiply-adds, as hipcc does by default: their accuracy is set by
+// the bf16 operands, not by the rounding of these sums.
 #pragma clang fp contract(fast)
 
 // error text shared with the Monte Carlo half of the library (pp_last_error)
