@@ -235,7 +235,9 @@ __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg
     return p;
 }
 
-__device__ __forceinline__ float clamp1(float x) { return fminf(fmaxf(x, -1.0f), 1.0f); }
+// x limited to [-1, 1] by one v_med3_f32 (fminf/fmaxf add an instruction that
+// canonicalises x first; x is never a NaN here)
+__device__ __forceinline__ float clamp1(float x) { return __builtin_amdgcn_fmed3f(x, -1.0f, 1.0f); }
 
 __device__ __forceinline__ float hg_cos(const Phase& p, float u) {
     const float t = p.hg_B * rcp_nr(fmaf(p.hg_D1, u, p.hg_D0));

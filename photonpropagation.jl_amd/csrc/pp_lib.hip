@@ -410,9 +410,10 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
     uint32_t ones;
     __asm__ volatile("v_mov_b32 %0, -1" : "=v"(ones));
     *reinterpret_cast<uint2*>(q.key + lane) = make_uint2(ones, ones);
-    // going: still stepping (active, and no hit found so far; steps after a
-    // found hit cannot change the earliest one)
-    bool going = active, stop = false;
+    // wave masks (scalar): going_m, the lanes still stepping (active, and no
+    // hit found so far; steps after a found hit cannot change the earliest
+    // one); stop_m, the lanes whose hit the last batch made known
+    uint64_t going_m = __ballot(active), stop_m = 0;
     uint32_t head = 0, count = 0;  // wave-uniform ring of pending jobs
     // every lane walks one pending job (lanes >= n none)
     auto walk_jobs = [&](uint32_t n) {
@@ -438,38 +439,40 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         wave_sync();
         if (found >= 0 && q.key[owner] == key) q.dt[owner] = dt;
         wave_sync();
-        stop = q.key[lane] != ~0ull;
+        stop_m = __ballot(q.key[lane] != ~0ull);
         head += n;
         if (head >= kQueue) head -= kQueue;
         count -= n;
     };
     // one loop for the steps and the final batches, so that the walk is
     // inlined once
+    // The lane conditions below are these scalar masks, applied with
+    // inverse ballots: no per-lane copy of a condition is kept or converted.
     int32_t step = 0;
     bool stepping = true;  // wave-uniform
     for (;;) {
-        going = going && !stop;
-        stepping = stepping && step < p.n_steps && __ballot(going) != 0;
+        going_m &= ~stop_m;
+        stepping = stepping && step < p.n_steps && going_m != 0;
         if (!stepping && count == 0) break;
-        const bool move = stepping && going;
-        if (stepping) ++step;
-        // r, step_size and next are set and read only where move holds (no
-        // register is set for the lanes that stay)
+        // r, step_size and next are set and read only while stepping (a
+        // uniform branch); every lane computes them, the moving lanes
+        // (move_m) use them (no register is set for the others)
+        uint64_t move_m = 0, m = 0;
         uint4 r;
         float step_size;
         f3 next;
-        bool need = false;
-        if (move) {
+        if (stepping) {
+            move_m = going_m;
+            ++step;
             r = ss.block((uint32_t)step);
             step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
             next = add_scaled(pos, step_size, dir);
             // a walk only for a segment near the grid; the walk clips it exactly
-            need = search.touches(pos, next);
+            m = search.touch_mask(pos, next) & move_m;
         }
-        const uint64_t m = __ballot(need);
         const uint32_t n_new = (uint32_t)__popcll(m);
         if (!stepping || count >= 64 || count + n_new > kQueue) walk_jobs(count < 64 ? count : 64);
-        if (need) {
+        if (__builtin_amdgcn_inverse_ballot_w64(m)) {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
             uint32_t slot = head + count + rank;
@@ -479,7 +482,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
                 make_float4(dir.x, dir.y, dir.z, __uint_as_float(lane | ((uint32_t)step << 6)));
         }
         count += n_new;
-        if (move) {
+        if (__builtin_amdgcn_inverse_ballot_w64(move_m)) {
             pos = next;
             const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
             dir = pp::update_direction(dir, ct, pp::u01(r.w));

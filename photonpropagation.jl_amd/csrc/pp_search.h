@@ -197,6 +197,21 @@ struct GridSearch {
         const bool z = ((a.z >= g.oz) | (b.z >= g.oz)) & ((a.z <= g.ez) | (b.z <= g.ez));
         return x & y & z;
     }
+    // touches() of every lane as a wave mask: each compare is a ballot of its
+    // own (one v_cmp writing a scalar mask) and the masks are combined by
+    // scalar instructions; as one lane condition the compiler merged the
+    // pairs of compares into minima and maxima, which need a canonicalising
+    // instruction per operand, and copied the result out of and back into a
+    // mask
+    __device__ __forceinline__ uint64_t touch_mask(f3 a, f3 b) const {
+        const uint64_t x = (__ballot(a.x >= g.ox) | __ballot(b.x >= g.ox)) &
+                           (__ballot(a.x <= g.ex) | __ballot(b.x <= g.ex));
+        const uint64_t y = (__ballot(a.y >= g.oy) | __ballot(b.y >= g.oy)) &
+                           (__ballot(a.y <= g.ey) | __ballot(b.y <= g.ey));
+        const uint64_t z = (__ballot(a.z >= g.oz) | __ballot(b.z >= g.oz)) &
+                           (__ballot(a.z <= g.ez) | __ballot(b.z <= g.ez));
+        return x & y & z;
+    }
     template <bool COUNT, bool COLS>
     __host__ __device__ __forceinline__ int walk(f3 pos, f3 dir, float step, float& dt,
                                                  Work& w) const {
