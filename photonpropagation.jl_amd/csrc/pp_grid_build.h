@@ -164,27 +164,62 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
         return false;
     }
     // Automatic edge (and, when cell_z_req is 0 too, shape): the candidate of
-    // least modelled walk cost per unit length. A segment of unit length with
-    // a uniform position and direction crosses (1/h + 1/h + 1/hz) / 2 cell
-    // faces on average and meets a box of surface S with probability S / (4 V)
-    // (Cauchy's formula, V the grid volume). A visit costs ~42 instructions
-    // (~37 in the 2-D walk of one layer); a box that is met costs 8 per member
-    // (the z-range filter) plus 40 per member left to test, a fraction
-    // (2 zpad + w) / height of them for a box of width w. Bigger cells are
-    // visited less, but once a cell holds targets far apart (two strings) its
-    // member box spans the gap and is met far more often; the model sees both.
+    // least modelled cost per unit of photon path. Inside the grid box, a
+    // segment of unit length with a uniform position and direction crosses
+    // (1/h + 1/h + 1/hz) / 2 cell faces on average and meets a box of surface
+    // S with probability S / (4 V) (Cauchy's formula, V the grid volume). A
+    // visit costs ~42 instructions (~37 in the 2-D walk of one layer); a box
+    // that is met costs 8 per member (the z-range filter) plus 40 per member
+    // left to test, a fraction (2 zpad + w) / height of them for a box of
+    // width w. Bigger cells are visited less, but once a cell holds targets far
+    // apart (two strings) its member box spans the gap and is met far more
+    // often. Each walk also costs ~75 instructions of set-up (clip, entry cell,
+    // batch bookkeeping), and a step walks when its end points' box meets the
+    // grid box: for steps of length L ~ 40 m (scattering lengths of deep
+    // water and ice) and isotropic directions, when it starts in the grid box
+    // grown by L/2 along each axis. Per unit of path in a region around the
+    // detector: cost = 75 prod(X_a + L/2) / L + V (visits and boxes per unit
+    // length), X_a the grid's sides - so a grid whose cells overshoot the
+    // targets' extent pays for its larger box. The candidates: 1-2.5x the edge
+    // of one cubic cell per target, and the edges of 1.5-2.5x it that divide
+    // the larger xy extent (with margins) into a whole number of cells exactly
+    // (smaller exact edges, one column per line of a dense lattice, measured
+    // slower than the model puts them: the member loop of a met box is bound
+    // by its LDS latency, which the model leaves out).
     const float base = std::cbrt(ext[0] * ext[1] * ext[2] / (float)n);
+    std::vector<float> cand = {1.0f * base, 1.25f * base, 1.5f * base, 1.75f * base,
+                               2.0f * base, 2.25f * base, 2.5f * base};
+    const float ext_xy = std::max(ext[0], ext[1]);
+    for (int k = 1; k <= 4096; ++k) {
+        // fixed point of h = (ext_xy + 4 m) / k with the margin m of try_build
+        // for cubic cells (hz = h) and for columns (hz the whole height), a
+        // little above it so that ceil gives k
+        float hk[2];
+        for (int col = 0; col < 2; ++col) {
+            float h = ext_xy / (float)k;
+            for (int it = 0; it < 3; ++it) {
+                const float hz = col ? ext[2] + 4.0f * (1e-3f * std::max(h, ext[2]) + 1e-3f) + 1e-2f : h;
+                h = (ext_xy + 4.0f * (1e-3f * std::max(h, hz) + 1e-3f)) / (float)k;
+            }
+            hk[col] = h * (1.0f + 4e-6f);
+        }
+        if (std::max(hk[0], hk[1]) < 1.5f * base) break;
+        for (float h : hk)
+            if (h >= 1.5f * base && h <= 2.5f * base) cand.push_back(h);
+    }
+    constexpr double kWalkSetup = 75.0, kStepLen = 40.0;
     bool found = false;
     double best = INFINITY;
     const int zmodes[2] = {zmode_req, -1};
     const int n_modes = cell_z_req == 0.0f ? 2 : 1;  // fully automatic: cubic or columns
     for (int mi = 0; mi < n_modes; ++mi) {
-        for (float f : {1.0f, 1.25f, 1.5f, 1.75f, 2.0f, 2.25f, 2.5f}) {
-            const float h = std::max(f * base, 1e-3f);
+        for (float h0 : cand) {
+            const float h = std::max(h0, 1e-3f);
             GridTables t;
             if (!try_build(h, zmodes[mi], hz_req, t)) continue;
             const GridDesc& g = t.g;
-            const double vol = (double)g.nx * g.h * g.ny * g.h * g.nz * g.hz;
+            const double sx = (double)g.nx * g.h, sy = (double)g.ny * g.h, sz = (double)g.nz * g.hz;
+            const double vol = sx * sy * sz;
             double met = 0.0;
             for (int c = 0; c < g.n_cells; ++c) {
                 const float4 a = t.cells[2 * (size_t)c], b = t.cells[2 * (size_t)c + 1];
@@ -197,8 +232,11 @@ inline bool build_grid_tables(const float* xyz, const float* radius, int n, floa
                 met += surf * cnt * (8.0 + 40.0 * frac);
             }
             const double per_visit = g.nz == 1 ? 37.0 : 42.0;
-            const double cost = per_visit * 0.5 * (2.0 / g.h + (g.nz == 1 ? 0.0 : 1.0 / g.hz)) +
-                                met / (4.0 * vol);
+            const double per_length = per_visit * 0.5 * (2.0 / g.h + (g.nz == 1 ? 0.0 : 1.0 / g.hz)) +
+                                      met / (4.0 * vol);
+            const double half = 0.5 * kStepLen;
+            const double cost = kWalkSetup * (sx + half) * (sy + half) * (sz + half) / kStepLen +
+                                per_length * vol;
             if (cost < best) {
                 best = cost;
                 out = std::move(t);

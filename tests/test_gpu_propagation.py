@@ -392,9 +392,10 @@ def test_grid_search_bit_exact_with_brute_force(P, gpu_lib, cell, cell_z):
 def test_lattice_of_4096_spheres(P, O, gpu_lib):
     """A 16 x 16 x 16 lattice of spheres (4096 targets: their records alone
     fill the grid's usual 64 KB of LDS, so the grid takes the whole LDS)
-    around a point source: the automatic grid (cubic cells, a 3-D walk) gives
-    the brute-force scan's hits bit for bit, both agree photon by photon with
-    the oracle, and the automatic search picks the grid."""
+    around a point source: a grid of 4 m cubic cells (a 3-D walk) gives the
+    brute-force scan's hits bit for bit, both agree photon by photon with the
+    oracle, and the automatic search picks the grid, whose cells (the model's
+    choice: one column per vertical line of spheres) give the same hits."""
     import ctypes
     import itertools
     medium = P.CascadiaMediumProperties()
@@ -408,6 +409,8 @@ def test_lattice_of_4096_spheres(P, O, gpu_lib):
         cfg = P._lib.pp_config()
         cfg.search = search
         cfg.deterministic_order = 1
+        if search == 2:
+            cfg.grid_cell = 4.0  # cubic cells (cell_z 0 with a set edge)
         _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
         with d:
             out[search] = gpu_run(P, d, wl)
@@ -420,15 +423,20 @@ def test_lattice_of_4096_spheres(P, O, gpu_lib):
                 print(f"grid cell {h.value:.2f} m, dims {list(dims)}, {ni.value} entries")
                 assert dims[2] > 1  # a 3-D grid, not columns
     (gb, nb), (gg, ng) = out[1], out[2]
-    _, d = P.make_hit_buffers(capacity=1 << 20)
+    cfg = P._lib.pp_config()
+    cfg.deterministic_order = 1
+    _, d = P.make_hit_buffers(capacity=1 << 20, config=cfg)
     with d:
-        gpu_run(P, d, wl)
-        info = ctypes.c_int32()
-        gpu_lib.pp_search_info(d.handle, ctypes.byref(info), None, None, None)
+        ga, na = gpu_run(P, d, wl)
+        info, h = ctypes.c_int32(), ctypes.c_float()
+        dims = (ctypes.c_int32 * 3)()
+        gpu_lib.pp_search_info(d.handle, ctypes.byref(info), ctypes.byref(h), dims, None)
         assert info.value == 2
-    assert nb == ng == 100_000 and len(gb["wavelength"]) > 1000
+        print(f"automatic grid: cell {h.value:.2f} m, dims {list(dims)}")
+    assert nb == ng == na == 100_000 and len(gb["wavelength"]) > 1000
     for k in gb:
         assert gb[k].tobytes() == gg[k].tobytes(), k
+        assert gb[k].tobytes() == ga[k].tobytes(), k
     r, rsim = oracle_run(P, O, wl)
     assert rsim == 100_000
     n, d = compare_photon_by_photon(gg, r, module_diff_frac=0.01)
