@@ -436,7 +436,9 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *                         DISC: a = radius)                      jl:353-404
  *   PP_FN_MIXED_HGES      in (g, b, hg_fraction, u_choice, u) out (cos)  the
  *                         kernel's branch-free MixedHGES sampler (HG with
- *                         probability hg_fraction, else Einstein-Smoluchowski)
+ *                         probability hg_fraction, else Einstein-Smoluchowski),
+ *                         on the random words of u_choice and u, which must be
+ *                         multiples of 2^-24 in [0, 1) (as the kernel's draws)
  * PP_FN_CASCADIA evaluates the context's medium (any kind): scattering
  * length, group velocity and phase index at wl. */
 enum pp_fn_id {

@@ -66,14 +66,16 @@ __device__ __forceinline__ float u01_open0(uint32_t x) {
     return (float)((x >> 8) + 1u) * 0x1p-24f;
 }
 
-// Identity of one photon's random stream.
+// Identity of one photon's random stream. block(word) is Philox4x32-10 of
+// the counter (word, pid_lo, pid_hi, src) under the key (k0, k1), evaluated
+// as StepStream (below) does: the word-independent half of rounds 1 and 2 is
+// the same for every block of a photon, so where the blocks are inlined into
+// one function it is computed once, and the word's own products are scalar.
 struct Stream {
     uint32_t k0, k1;    // seed
     uint32_t pid_lo, pid_hi;
     uint32_t src;
-    __device__ __forceinline__ uint4 block(uint32_t word) const {
-        return philox(make_uint4(word, pid_lo, pid_hi, src), k0, k1);
-    }
+    __device__ __forceinline__ uint4 block(uint32_t word) const;
 };
 
 // The step draws of one photon: Stream::block(word), bit for bit, with the
@@ -113,6 +115,10 @@ struct StepStream {
         return philox_rounds<7>(c, k0 + 3u * kPhiloxW0, k1 + 3u * kPhiloxW1);
     }
 };
+
+__device__ __forceinline__ uint4 Stream::block(uint32_t word) const {
+    return StepStream(*this).block(word);
+}
 
 struct f3 {
     float x, y, z;
@@ -216,6 +222,10 @@ struct Phase {
     float es_Q, es_P3, es_P3cube;
     // simplified Liu: p(cos) ~ (1 + cos)^(2g/(1-g)), cos = 2 u^((1-g)/(1+g)) - 1
     float sl_exp;
+    // the HG/ES mixture on the random words (hg_es_cos_words): hg_D1 2^-24,
+    // and the HG choice u < hg_fraction as (word >> 8) < hg_words
+    float hg_D1w;
+    uint32_t hg_words;
 };
 
 __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg_fraction) {
@@ -232,6 +242,11 @@ __host__ __device__ inline Phase make_phase(int kind, float g, float b, float hg
     p.es_P3 = 1.0f / b;
     p.es_P3cube = p.es_P3 * p.es_P3 * p.es_P3;
     p.sl_exp = (1.0f - g) / (1.0f + g);
+    p.hg_D1w = p.hg_D1 * 0x1p-24f;
+    // u01(w) = n 2^-24 < f  <=>  n < f 2^24  <=>  n < ceil(f 2^24) (n an integer;
+    // f 2^24 is exact)
+    const float f24 = hg_fraction * 0x1p24f;
+    p.hg_words = !(f24 > 0.0f) ? 0u : f24 >= 0x1p24f ? (1u << 24) : (uint32_t)ceilf(f24);
     return p;
 }
 
@@ -271,10 +286,33 @@ __device__ __forceinline__ float hg_es_cos(const Phase& p, float u_choice, float
     return clamp1(hg ? fmaf(-p.hg_C, t * t, p.hg_A) : (q > 0.0f ? -x : x));
 }
 
+// hg_es_cos(p, u01(w_choice), u01(w_u)), bit for bit, from the two random
+// words: the choice as an integer compare, and the 2^-24 of u folded into the
+// constants it multiplies (-2 u = -2^-23 n and D1 u = (D1 2^-24) n exactly, so
+// each FMA rounds the same value)
+__device__ __forceinline__ float hg_es_cos_words(const Phase& p, uint32_t w_choice, uint32_t w_u) {
+    const bool hg = (w_choice >> 8) < p.hg_words;
+    const float n = (float)(w_u >> 8);
+    const float q = p.es_Q * fmaf(-0x1p-23f, n, 1.0f);
+    const float hq = 0.5f * fabsf(q);
+    const float w = cbrt_pos(hq + sqrt_hw(fmaf(hq, hq, p.es_P3cube)));
+    const float r = rcp_nr(hg ? fmaf(p.hg_D1w, n, p.hg_D0) : w);
+    const float t = p.hg_B * r;
+    const float x = fmaf(-p.es_P3, r, w);
+    return clamp1(hg ? fmaf(-p.hg_C, t * t, p.hg_A) : (q > 0.0f ? -x : x));
+}
+
 __device__ __forceinline__ float phase_cos(const Phase& p, float u_choice, float u) {
     if (p.kind == PP_PHASE_HG) return hg_cos(p, u);
     if (p.kind == PP_PHASE_MIXED_HG_SL) return u_choice < p.hg_fraction ? hg_cos(p, u) : sl_cos(p, u);
     return hg_es_cos(p, u_choice, u);
+}
+
+// phase_cos(p, u01(w_choice), u01(w_u)), the form the propagation kernels call
+__device__ __forceinline__ float phase_cos_words(const Phase& p, uint32_t w_choice, uint32_t w_u) {
+    if (p.kind == PP_PHASE_HG || p.kind == PP_PHASE_MIXED_HG_SL)
+        return phase_cos(p, u01(w_choice), u01(w_u));
+    return hg_es_cos_words(p, w_choice, w_u);
 }
 
 // update_direction   src/photon_prop_cuda.jl:280-298

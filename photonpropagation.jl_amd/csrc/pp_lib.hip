@@ -326,7 +326,7 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
             break;
         }
         move_by(pos, dist, time, dir, step_size, inv_c);
-        const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
+        const float ct = pp::phase_cos_words(phase, r.y, r.z);
         dir = pp::update_direction(dir, ct, pp::u01(r.w));
     }
     h = HitRec{pos, dir, st.dir, st.wl, time, dist, module_ix, n_at, pid};
@@ -484,7 +484,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         count += n_new;
         if (__builtin_amdgcn_inverse_ballot_w64(move_m)) {
             pos = next;
-            const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
+            const float ct = pp::phase_cos_words(phase, r.y, r.z);
             dir = pp::update_direction(dir, ct, pp::u01(r.w));
         }
     }
@@ -504,7 +504,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
             const uint4 r = ss.block((uint32_t)i);
             const float step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
             move_by(rp, rdist, rt, rd, step_size, inv_c);
-            const float ct = pp::phase_cos(phase, pp::u01(r.y), pp::u01(r.z));
+            const float ct = pp::phase_cos_words(phase, r.y, r.z);
             rd = pp::update_direction(rd, ct, pp::u01(r.w));
         }
         move_by(rp, rdist, rt, rd, dtk, inv_c);
@@ -814,9 +814,11 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         break;
     }
     case PP_FN_MIXED_HGES: {
-        // the sampler phase_cos runs for a MixedHGES medium in the kernel
+        // the sampler the kernels run for a MixedHGES medium, on the random
+        // words whose u01 are x[3], x[4] (multiples of 2^-24 in [0, 1))
         pp::Phase ph = pp::make_phase(PP_PHASE_MIXED_HG_ES, x[0], x[1], x[2]);
-        y[0] = pp::phase_cos(ph, x[3], x[4]);
+        const uint32_t wc = (uint32_t)(x[3] * 0x1p24f) << 8, wu = (uint32_t)(x[4] * 0x1p24f) << 8;
+        y[0] = pp::phase_cos_words(ph, wc, wu);
         break;
     }
     case PP_FN_PLANE_ISEC: {

@@ -308,15 +308,19 @@ def test_gamma_sampler_same_draws(P, O, dev):
 @pytest.mark.parametrize("g_", [0.95, 0.5])
 def test_mixed_hges_kernel_sampler(P, O, dev, g_):
     """The branch-free HG/ES mixture the propagation kernel calls (both
-    samplers share one Newton reciprocal) against the oracle's MixedHGES
-    (branch choice, then the cosine)."""
+    samplers share one Newton reciprocal; it takes the random words, u =
+    k 2^-24) against the oracle's MixedHGES (branch choice, then the cosine),
+    with the ends and the choice threshold among the draws."""
     n = 200_000
     x = np.zeros((n, 5), np.float32)
     x[:, 0] = g_
     x[:, 1] = 0.835
     x[:, 2] = 0.8
-    x[:, 3] = RNG.random(n, dtype=np.float32)
-    x[:, 4] = RNG.random(n, dtype=np.float32)
+    x[:, 3] = RNG.integers(0, 1 << 24, n).astype(np.float32) * np.float32(2.0**-24)
+    x[:, 4] = RNG.integers(0, 1 << 24, n).astype(np.float32) * np.float32(2.0**-24)
+    t = np.float32(np.ceil(np.float64(np.float32(0.8)) * 2.0**24)) * np.float32(2.0**-24)
+    x[:6, 3] = [0.0, 1.0 - 2.0**-24, t - np.float32(2.0**-24), t, t + np.float32(2.0**-24), 0.5]
+    x[:6, 4] = [0.0, 1.0 - 2.0**-24, 0.5, 2.0**-24, 0.25, 0.75]
     g, r = check_vec(P, O, dev, "MIXED_HGES", x, f"mixed_hges g={g_}")
     hg = x[:, 3] < x[:, 2]
     assert 0.79 < hg.mean() < 0.81
