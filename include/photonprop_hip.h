@@ -159,7 +159,10 @@ enum pp_batch_mode {
 };
 
 typedef struct pp_config {
-    int32_t block_size;         /* threads per block, multiple of 64; 0 = 256 */
+    int32_t block_size;         /* threads per block, multiple of 64; 0 = chosen per
+                                   call: of 256, 384, 512, 768, 1024 the one that keeps
+                                   the most waves resident (LDS of tables and walk
+                                   queues), the smallest on a tie */
     int32_t waves_per_simd;     /* grid sizing: blocks = CUs*4*wps*64/block; 0 = auto */
     int32_t search;             /* enum pp_search_kind                         */
     float grid_cell;            /* grid cell edge (m); 0 = auto: the edge (and,

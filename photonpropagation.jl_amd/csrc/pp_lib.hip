@@ -925,6 +925,7 @@ struct pp_ctx {
     int64_t work_steps = 0, work_tests = 0, work_cells = 0, work_walks = 0;
     float last_ms = 0.0f;
     int32_t n_launch = 0;
+    int block = 256;  // threads per block of the last launch (cfg.block_size, or chosen for 0)
     bool perm_valid = false;
     std::vector<int64_t> perm;
     // pinned host staging of hit records for pp_copy_hits
@@ -1025,9 +1026,9 @@ int build_grid(pp_ctx* c, float cell_req, float cell_z_req) {
 // Defaults and range checks of a pp_config.
 int normalise_config(pp_config& c, const char* who) {
     const std::string w(who);
-    if (c.block_size == 0) c.block_size = 256;
-    if (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64)
-        return fail(PP_ERR_INVALID, w + ": block_size must be a multiple of 64 in [64,1024]");
+    // block_size 0: chosen per call from the LDS the launch needs (pp_propagate)
+    if (c.block_size != 0 && (c.block_size < 64 || c.block_size > 1024 || c.block_size % 64))
+        return fail(PP_ERR_INVALID, w + ": block_size must be 0 or a multiple of 64 in [64,1024]");
     if (c.waves_per_simd < 0 || c.waves_per_simd > 32)
         return fail(PP_ERR_INVALID, w + ": waves_per_simd must be in [0,32]");
     if (c.search < PP_SEARCH_AUTO || c.search > PP_SEARCH_GRID)
@@ -1049,6 +1050,8 @@ int normalise_config(pp_config& c, const char* who) {
 int max_resident_blocks(pp_ctx* c, const void* fn, int block, size_t lds, int64_t& out) {
     int occ = 0;
     PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
+    // LDS is allocated in 512-byte units per workgroup, which the query leaves out
+    if (lds > 0) occ = std::min<int>(occ, (int)((size_t)kMaxLdsBytes / ((lds + 511) & ~(size_t)511)));
     if (occ < 1) return fail(PP_ERR_INVALID, "pp_propagate: kernel does not fit one block per CU");
     out = (int64_t)occ * c->n_cu;
     if (c->cfg.waves_per_simd > 0)
@@ -1072,6 +1075,19 @@ int ensure_batch_table(pp_ctx* c, size_t bytes) {
     return PP_OK;
 }
 
+// The persistent kernel of a search kind and phase: the one whose occupancy
+// chooses the automatic block size.
+const void* batch_kernel_fn(int skind, int phase) {
+    using BatchFn = void (*)(KParams, BatchParams);
+#define PP_K(PH, S) propagate_batch_kernel<PH, S, false>
+#define PP_K3(S) {PP_K(PP_PHASE_HG, S), PP_K(PP_PHASE_MIXED_HG_ES, S), PP_K(PP_PHASE_MIXED_HG_SL, S)}
+    static const BatchFn kKernels[4][3] = {PP_K3(kSearchBrute), PP_K3(kSearchGrid),
+                                           PP_K3(kSearchMixed), PP_K3(kSearchGridGlobal)};
+#undef PP_K3
+#undef PP_K
+    return (const void*)kKernels[skind][phase];
+}
+
 // One launch per source, as src/photon_prop_cuda.jl:925-944.
 int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
                       const std::vector<uint64_t>& nph, int skind, int phase, bool count, size_t lds) {
@@ -1087,7 +1103,7 @@ int launch_per_source(pp_ctx* c, KParams kp, const pp_source* sources,
 #undef PP_K3
 #undef PP_K
     const KernelFn kern = kKernels[count][skind][phase];
-    const int block = c->cfg.block_size;
+    const int block = c->block;
     int64_t max_blocks = 0;
     if (int rc = max_resident_blocks(c, (const void*)kern, block, lds, max_blocks)) return rc;
     // the source list on the device (the kernels read their source from it)
@@ -1127,7 +1143,7 @@ int launch_batch(pp_ctx* c, const KParams& kp, const pp_source* sources,
 #undef PP_K3
 #undef PP_K
     const BatchFn kern = kKernels[count][skind][phase];
-    const int block = c->cfg.block_size;
+    const int block = c->block;
     const size_t n = nph.size();
     uint64_t total_ph = 0;
     for (uint64_t v : nph) total_ph += v;
@@ -1550,7 +1566,34 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
                                     grid ? ctx->grid.n_items : 0, ctx->spec_n, grid);
     // the grid kernels' per-wave walk queues follow the tables
     const uint32_t queue_off = (uint32_t)((lds + 15) & ~(size_t)15);
-    if (grid) lds = queue_off + (size_t)(ctx->cfg.block_size / 64) * kWaveQueueBytes;
+    // block size: the set one or, for 0, the one of 256, 384, 512, 768 and 1024
+    // threads with the most waves resident per CU: the runtime's occupancy
+    // query (registers: 6 waves per SIMD) and the LDS a workgroup needs (the
+    // tables once and, with a grid, a walk queue per wave) in 512-byte
+    // allocation units, which the query does not apply (it reports three
+    // workgroups of 54320 bytes, and they run as two); the smallest on a tie.
+    // (1000 targets: 768 threads, two workgroups of 12 waves, where 256 and
+    // 512 threads leave 16 waves.)
+    int block = ctx->cfg.block_size;
+    if (block == 0) {
+        const void* fn = batch_kernel_fn(skind, ctx->medium.phase.kind);
+        int best_waves = 0;
+        for (int b : {256, 384, 512, 768, 1024}) {
+            const size_t need = grid ? queue_off + (size_t)(b / 64) * kWaveQueueBytes : lds;
+            if (need > (size_t)kMaxLdsBytes) continue;
+            int occ = 0;
+            PP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, b, need));
+            const size_t alloc = (need + 511) & ~(size_t)511;
+            occ = std::min<int>(occ, (int)((size_t)kMaxLdsBytes / alloc));
+            if (occ * (b / 64) > best_waves) {
+                best_waves = occ * (b / 64);
+                block = b;
+            }
+        }
+        if (block == 0) block = 64;  // the LDS check below reports what does not fit
+    }
+    ctx->block = block;
+    if (grid) lds = queue_off + (size_t)(block / 64) * kWaveQueueBytes;
     if (lds > (size_t)kMaxLdsBytes)
         return fail(PP_ERR_INVALID, "pp_propagate: targets, tables and walk queues exceed the LDS "
                                     "of a workgroup (a smaller block_size may fit)");
@@ -1671,14 +1714,14 @@ int pp_autotune(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t s
     std::vector<float> cells = {0.0f};
     if (grid)
         for (float f : {0.6f, 0.8f, 0.9f, 1.1f, 1.25f}) cells.push_back(f * h_auto);
-    const int blocks[] = {64, 128, 256, 512, 1024};
+    const int blocks[] = {64, 128, 256, 384, 512, 768, 1024};
     const int chunks[] = {0, 64, 128, 256, 512, 1024};
     // cell shapes: cubic, columns, and cells twice as tall as wide (set when tried)
     float shapes[] = {0.0f, -1.0f, 0.0f};
     for (int pass = 0; pass < 2; ++pass) {
         for (int knob = 0; knob < 4; ++knob) {
             if (knob >= 2 && !grid) continue;
-            const size_t n_cand = knob == 0 ? 5 : knob == 1 ? 6 : knob == 2 ? cells.size() : 3;
+            const size_t n_cand = knob == 0 ? 7 : knob == 1 ? 6 : knob == 2 ? cells.size() : 3;
             if (knob == 3) shapes[2] = 2.0f * (cur.grid_cell > 0.0f ? cur.grid_cell : h_auto);
             for (size_t k = 0; k < n_cand; ++k) {
                 pp_config c = cur;
