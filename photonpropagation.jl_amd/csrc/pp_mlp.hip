@@ -18,6 +18,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "photonprop_hip.h"
@@ -111,7 +112,15 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-    f32x16 acc = {};
+    // acc[r] holds row (r/4)*8 + (lane/32)*4 + r%4, column lane%32 of the wave's
+    // tile; a bias epilogue starts the sum from the column's bias (the first
+    // product adds to it, as in mlp_fused_kernel)
+    const int n = n0 + wn + (lane & 31);
+    const float bias0 =
+        (EPI == EPI_BIAS_F32 || EPI == EPI_BIAS_SILU) && n < g.N ? g.bias[n] : 0.0f;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = bias0;
     const int k_lo = blockIdx.z * g.k_chunk;
     const int k_hi = k_lo + g.k_chunk < g.K ? k_lo + g.k_chunk : g.K;
     float* const C = g.C ? g.C + blockIdx.z * g.c_stride : nullptr;
@@ -128,10 +137,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         }
         __syncthreads();
     }
-    // acc[r] holds row (r/4)*8 + (lane/32)*4 + r%4, column lane%32 of the wave's tile
-    const int n = n0 + wn + (lane & 31);
     if (n >= g.N) return;
-    const float bias = (EPI == EPI_BIAS_F32 || EPI == EPI_BIAS_SILU) ? g.bias[n] : 0.0f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int m = m0 + wm + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
@@ -140,9 +146,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         if (EPI == EPI_F32) {
             C[(size_t)m * g.ldc + n] = v;
         } else if (EPI == EPI_BIAS_F32) {
-            C[(size_t)m * g.ldc + n] = v + bias;
+            C[(size_t)m * g.ldc + n] = v;
         } else if (EPI == EPI_BIAS_SILU) {
-            const float z = v + bias;
+            const float z = v;
             if (C) C[(size_t)m * g.ldc + n] = z;
             g.H[(size_t)m * g.ldh + n] = (bf16)(z * sigmoid(z));
         } else {
@@ -515,13 +521,26 @@ __device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act
     const int m0 = MODE == kFOutput ? 32 * w : 0;         // first row
     const bf16x8* wsrc = reinterpret_cast<const bf16x8*>(f.W[l]) + (size_t)t0 * NF * 64 + lane;
     const bf16* bsrc = act + (m0 + (lane & 31)) * kFActLd + kl;
-    f32x16 acc[NT][MT];
+    // the sums start from the biases: acc[nt][mt][4g + j] belongs to column
+    // 32(t0 + nt) + 8g + (lane/32)*4 + j of any row tile, so one block of
+    // biases per column tile is the first product's accumulator input
+    f32x16 bb[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = f32x16{};
-#pragma unroll 1
-    for (int i0 = 0; i0 < NF; i0 += NB) {
+        for (int g = 0; g < 4; ++g) {
+            const float4 b4 = *reinterpret_cast<const float4*>(bias_l + 32 * (t0 + nt) + 8 * g +
+                                                               (lane >> 5) * 4);
+            bb[nt][4 * g] = b4.x;
+            bb[nt][4 * g + 1] = b4.y;
+            bb[nt][4 * g + 2] = b4.z;
+            bb[nt][4 * g + 3] = b4.w;
+        }
+    f32x16 acc[NT][MT];
+    // one batch of NB weight fragments from fragment i0; FIRST: the batch at
+    // fragment 0, whose first products take the bias blocks
+    auto batch = [&](int i0, auto first) {
+        constexpr bool FIRST = decltype(first)::value;
         bf16x8 wa[NT][NB];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
@@ -542,11 +561,14 @@ __device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act
             for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
-                    acc[nt][mt] =
-                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[nt][i], b[mt], acc[nt][mt], 0, 0, 0);
+                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        wa[nt][i], b[mt], (FIRST && i == 0) ? bb[nt] : acc[nt][mt], 0, 0, 0);
         }
         __builtin_amdgcn_s_setprio(0);
-    }
+    };
+    batch(0, std::true_type{});
+#pragma unroll 1
+    for (int i0 = NB; i0 < NF; i0 += NB) batch(i0, std::false_type{});
     __syncthreads();
     // acc[nt][mt][4g + j]: column 32(t0 + nt) + 8g + (lane/32)*4 + j, row m0 + 32mt + lane%32
 #pragma unroll
@@ -554,14 +576,12 @@ __device__ __forceinline__ void fused_layer(const FusedArgs& f, int l, bf16* act
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int n0 = 32 * (t0 + nt) + 8 * g + (lane >> 5) * 4;
-            const float4 b4 = *reinterpret_cast<const float4*>(bias_l + n0);
-            const float bias[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const int m = m0 + 32 * mt + (lane & 31);
                 float z[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) z[j] = acc[nt][mt][4 * g + j] + bias[j];
+                for (int j = 0; j < 4; ++j) z[j] = acc[nt][mt][4 * g + j];
                 if (MODE == kFOutput) {
                     *reinterpret_cast<float4*>(reinterpret_cast<float*>(act) + m * 32 + n0) =
                         make_float4(z[0], z[1], z[2], z[3]);
