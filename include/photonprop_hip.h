@@ -441,7 +441,8 @@ int pp_work_counts(pp_ctx* ctx, int64_t* steps, int64_t* sphere_tests, int64_t* 
  *                         kernel's branch-free MixedHGES sampler (HG with
  *                         probability hg_fraction, else Einstein-Smoluchowski),
  *                         on the random words of u_choice and u, which must be
- *                         multiples of 2^-24 in [0, 1) (as the kernel's draws)
+ *                         multiples of 2^-24 in [0, 1) (as the kernel's draws);
+ *                         NaN if it differs from the float form's bits
  * PP_FN_CASCADIA evaluates the context's medium (any kind): scattering
  * length, group velocity and phase index at wl. */
 enum pp_fn_id {

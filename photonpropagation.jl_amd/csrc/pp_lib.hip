@@ -819,6 +819,9 @@ __global__ void __launch_bounds__(256) eval_fn_kernel(EvalParams e) {
         pp::Phase ph = pp::make_phase(PP_PHASE_MIXED_HG_ES, x[0], x[1], x[2]);
         const uint32_t wc = (uint32_t)(x[3] * 0x1p24f) << 8, wu = (uint32_t)(x[4] * 0x1p24f) << 8;
         y[0] = pp::phase_cos_words(ph, wc, wu);
+        // the words form must give the float form's bits: NaN where it does not
+        if (__float_as_uint(y[0]) != __float_as_uint(pp::hg_es_cos(ph, pp::u01(wc), pp::u01(wu))))
+            y[0] = __uint_as_float(0x7fc00000u);
         break;
     }
     case PP_FN_PLANE_ISEC: {
