@@ -493,27 +493,18 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
     const bool hit = active && key != ~0ull;
     if (COUNT && active) n_steps_done += hit ? (unsigned long long)(key >> 16) : (unsigned long long)p.n_steps;
     if (hit) {
-        // the photon's emission and steps again, up to the one that hits
-        const PhotonInit st = init_photon(p, src, spec, s);
-        const float inv_c = st.mv.inv_c_grp;
-        const int32_t k = (int32_t)(key >> 16);
-        const float dtk = q.dt[lane];
-        f3 rp = st.pos, rd = st.dir;
-        float rt = st.time, rdist = 0.0f;
-        for (int32_t i = 1; i < k; ++i) {
-            const uint4 r = ss.block((uint32_t)i);
-            const float step_size = pp::step_length(pp::u01_open0(r.x), ln2_sca);
-            move_by(rp, rdist, rt, rd, step_size, inv_c);
-            const float ct = pp::phase_cos_words(phase, r.y, r.z);
-            rd = pp::update_direction(rd, ct, pp::u01(r.w));
-        }
-        move_by(rp, rdist, rt, rd, dtk, inv_c);
-        // the photon id recomputed from the wave's first id (scalar) and the
-        // lane: a per-lane copy kept for the rare hit was spilled per photon
+        // the record holds the photon id, module and step, and in its distance
+        // field the distance within that step; replay_hits_kernel completes it
+        // (emission again, steps 1..k-1, the partial step k), so that neither
+        // the emission's state nor a second inlined emission stays in this
+        // kernel's registers. The photon id is recomputed from the wave's first
+        // id (scalar) and the lane.
         uint64_t first = wave_first;
         __asm__ volatile("" : "+s"(first));
         const uint64_t id = first + lane;
-        h = HitRec{rp, rd, st.dir, st.wl, rt, rdist, (int32_t)(key & 0xFFFFu) + 1, k, id};
+        const f3 zero = pp::make3(0.0f, 0.0f, 0.0f);
+        h = HitRec{zero, zero, zero, 0.0f, 0.0f, q.dt[lane], (int32_t)(key & 0xFFFFu) + 1,
+                   (int32_t)(key >> 16), id};
     }
     return hit;
 }
@@ -704,6 +695,46 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
         if (__ballot(overflow)) break;  // the call fails; stop taking chunks
     }
     flush_counts<COUNT>(p, n_done, wave_done, n_steps_done, work);
+}
+
+// Completion of the hit records of the deferred-walk kernels (transport_group):
+// a record holds the photon id, module, step k and source, and in its distance
+// field the distance d within step k. One lane per record emits the photon
+// again and replays steps 1..k-1 and then d with transport_photon's arithmetic
+// on the same random stream (contraction off: the same bits as the one-photon
+// schedule), and writes position, directions, wavelength, time and distance.
+// sources: the call's source list on the device.
+__global__ void __launch_bounds__(256) replay_hits_kernel(KParams p, const pp_source* sources) {
+    const pp::Phase phase = p.med.phase;
+    const unsigned long long n_hits = p.counters->n_hits;
+    const uint64_t n = n_hits < (unsigned long long)p.hits.capacity ? n_hits : p.hits.capacity;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float4* r = reinterpret_cast<float4*>(p.hits.rec + i);
+        const float4 r2 = r[2], r3 = r[3];
+        const uint64_t id = (uint64_t)__float_as_uint(r3.x) | ((uint64_t)__float_as_uint(r3.y) << 32);
+        const uint64_t pid = id & (kMaxPhotonsPerSource - 1);
+        const int32_t k = __float_as_int(r3.z);
+        const uint32_t si = __float_as_uint(r3.w);
+        const pp_source src = sources[si];
+        const pp::Stream s{p.k0, p.k1, (uint32_t)pid, (uint32_t)(pid >> 32), si};
+        const pp::StepStream ss(s);
+        const PhotonInit st = init_photon(p, src, p.spec, s);
+        const float ln2_sca = st.mv.sca_len * pp::kLn2, inv_c = st.mv.inv_c_grp;
+        f3 rp = st.pos, rd = st.dir;
+        float rt = st.time, rdist = 0.0f;
+        for (int32_t j = 1; j < k; ++j) {
+            const uint4 w = ss.block((uint32_t)j);
+            const float step_size = pp::step_length(pp::u01_open0(w.x), ln2_sca);
+            move_by(rp, rdist, rt, rd, step_size, inv_c);
+            const float ct = pp::phase_cos_words(phase, w.y, w.z);
+            rd = pp::update_direction(rd, ct, pp::u01(w.w));
+        }
+        move_by(rp, rdist, rt, rd, r2.w, inv_c);
+        r[0] = make_float4(rp.x, rp.y, rp.z, rd.x);
+        r[1] = make_float4(rd.y, rd.z, st.dir.x, st.dir.y);
+        r[2] = make_float4(st.dir.z, st.wl, rt, rdist);
+    }
 }
 
 // Zeroes the counters in a captured graph (a kernel node, not a memset node).
@@ -1641,6 +1672,12 @@ int pp_propagate(pp_ctx* ctx, const pp_source* sources, int n_sources, uint64_t 
     } else {
         if (int rc = launch_batch(ctx, kp, sources, nph, skind, phase, count, lds, mode == PP_BATCH_GRAPH))
             return rc;
+    }
+    // the grid kernels leave their hit records to be completed
+    if (skind == kSearchGrid || skind == kSearchGridGlobal) {
+        hipLaunchKernelGGL(replay_hits_kernel, dim3((unsigned)(4 * ctx->n_cu)), dim3(256), 0,
+                           ctx->stream, kp, (const pp_source*)ctx->d_batch);
+        PP_HIP(hipGetLastError());
     }
     PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     Counters hc{};
