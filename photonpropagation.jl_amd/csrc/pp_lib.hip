@@ -345,10 +345,12 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
 // photon's LDS slot with an atomic minimum; the earliest step wins, which is
 // the step at which transport_photon stops (its later steps were never taken
 // by the reference and their jobs only lose the minimum); a photon stops
-// stepping once a hit of its own is known. At the end the few photons with a
-// hit are emitted again and replay their steps up to it (same random stream,
-// same arithmetic) for the hit record; no emission state stays live across
-// the steps. The result is that of transport_photon bit for bit.
+// stepping once a hit of its own is known. A hit's record holds the photon id,
+// module, step and the distance within it; replay_hits_kernel then emits the
+// few photons with a hit again and replays their steps up to it (same random
+// stream, same arithmetic) for the rest of the record, so no emission state
+// stays live across the steps. The result is that of transport_photon bit
+// for bit.
 // ---------------------------------------------------------------------------
 // Jobs per wave: a batch of up to 64 runs at the start of a step whenever 64
 // or more are pending or the step's new jobs would not fit, so at most kQueue
