@@ -102,8 +102,9 @@ __host__ __device__ __forceinline__ float pinned(float x) {
     return x;
 }
 
+// v limited to [0, hi], hi >= 0 (one v_med3_i32 on the device)
 __host__ __device__ __forceinline__ int clampi(int v, int hi) {
-    return v < 0 ? 0 : (v > hi ? hi : v);
+    return min(max(v, 0), hi);
 }
 
 __host__ __device__ __forceinline__ uint32_t float_bits(float x) {
