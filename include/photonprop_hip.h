@@ -174,7 +174,8 @@ typedef struct pp_config {
                                    tests and grid-cell visits (pp_work_counts) */
     int32_t batch_mode;         /* enum pp_batch_mode                          */
     int32_t chunk_photons;      /* photons per work chunk (multiple of 64);
-                                   0 = auto                                    */
+                                   0 = auto: about 32 chunks per resident
+                                   wave, at least 256 photons                  */
     float grid_cell_z;          /* grid cell edge along z (m): 0 = cubic cells
                                    (automatic shape if grid_cell is 0 too),
                                    < 0 = one layer over the targets' height
