@@ -356,13 +356,12 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
 // or more are pending or the step's new jobs would not fit, so at most kQueue
 // are pending after any step.
 constexpr uint32_t kQueue = 96;
-constexpr uint32_t kWaveQueueBytes = kQueue * 32 + 64 * 8 + 64 * 4;
+constexpr uint32_t kWaveQueueBytes = kQueue * 32 + 64 * 8;
 constexpr int32_t kDeferredMaxSteps = (1 << 26) - 1;  // step index of a job: 26 bits
 
 struct WaveQueue {
     float4* job;               // 2 per job: (position, step length), (direction, lane | step << 6)
     unsigned long long* key;   // per lane: earliest (step << 16 | target) hit, ~0 if none
-    float* dt;                 // per lane: distance to that hit within its step
 };
 
 __device__ __forceinline__ WaveQueue wave_queue(const KParams& p) {
@@ -371,7 +370,6 @@ __device__ __forceinline__ WaveQueue wave_queue(const KParams& p) {
     WaveQueue q;
     q.job = reinterpret_cast<float4*>(base);
     q.key = reinterpret_cast<unsigned long long*>(base + kQueue * 32);
-    q.dt = reinterpret_cast<float*>(base + kQueue * 32 + 64 * 8);
     return q;
 }
 
@@ -439,8 +437,6 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
             }
         }
         wave_sync();
-        if (found >= 0 && q.key[owner] == key) q.dt[owner] = dt;
-        wave_sync();
         stop_m = __ballot(q.key[lane] != ~0ull);
         head += n;
         if (head >= kQueue) head -= kQueue;
@@ -495,9 +491,9 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
     const bool hit = active && key != ~0ull;
     if (COUNT && active) n_steps_done += hit ? (unsigned long long)(key >> 16) : (unsigned long long)p.n_steps;
     if (hit) {
-        // the record holds the photon id, module and step, and in its distance
-        // field the distance within that step; replay_hits_kernel completes it
-        // (emission again, steps 1..k-1, the partial step k), so that neither
+        // the record holds the photon id, module and step; replay_hits_kernel
+        // completes it (emission again, steps 1..k-1, the sphere test of step
+        // k for the distance within it), so that neither
         // the emission's state nor a second inlined emission stays in this
         // kernel's registers. The photon id is recomputed from the wave's first
         // id (scalar) and the lane.
@@ -505,7 +501,7 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
         __asm__ volatile("" : "+s"(first));
         const uint64_t id = first + lane;
         const f3 zero = pp::make3(0.0f, 0.0f, 0.0f);
-        h = HitRec{zero, zero, zero, 0.0f, 0.0f, q.dt[lane], (int32_t)(key & 0xFFFFu) + 1,
+        h = HitRec{zero, zero, zero, 0.0f, 0.0f, 0.0f, (int32_t)(key & 0xFFFFu) + 1,
                    (int32_t)(key >> 16), id};
     }
     return hit;
@@ -700,11 +696,12 @@ __global__ void PP_PROPAGATE_ATTRS propagate_batch_kernel(KParams p, BatchParams
 }
 
 // Completion of the hit records of the deferred-walk kernels (transport_group):
-// a record holds the photon id, module, step k and source, and in its distance
-// field the distance d within step k. One lane per record emits the photon
-// again and replays steps 1..k-1 and then d with transport_photon's arithmetic
-// on the same random stream (contraction off: the same bits as the one-photon
-// schedule), and writes position, directions, wavelength, time and distance.
+// a record holds the photon id, module (target + 1), step k and source. One
+// lane per record emits the photon again, replays steps 1..k-1 with
+// transport_photon's arithmetic on the same random stream (contraction off:
+// the same bits as the one-photon schedule), takes step k's distance to the
+// target from the walk's own sphere test on the same operands, and writes
+// position, directions, wavelength, time and distance.
 // sources: the call's source list on the device.
 __global__ void __launch_bounds__(256) replay_hits_kernel(KParams p, const pp_source* sources) {
     const pp::Phase phase = p.med.phase;
@@ -713,9 +710,10 @@ __global__ void __launch_bounds__(256) replay_hits_kernel(KParams p, const pp_so
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         float4* r = reinterpret_cast<float4*>(p.hits.rec + i);
-        const float4 r2 = r[2], r3 = r[3];
+        const float4 r3 = r[3];
         const uint64_t id = (uint64_t)__float_as_uint(r3.x) | ((uint64_t)__float_as_uint(r3.y) << 32);
         const uint64_t pid = id & (kMaxPhotonsPerSource - 1);
+        const int32_t target = (int32_t)((id >> kPidBits) & 0xFFFFu) - 1;
         const int32_t k = __float_as_int(r3.z);
         const uint32_t si = __float_as_uint(r3.w);
         const pp_source src = sources[si];
@@ -732,7 +730,12 @@ __global__ void __launch_bounds__(256) replay_hits_kernel(KParams p, const pp_so
             const float ct = pp::phase_cos_words(phase, w.y, w.z);
             rd = pp::update_direction(rd, ct, pp::u01(w.w));
         }
-        move_by(rp, rdist, rt, rd, r2.w, inv_c);
+        // step k, cut at the target: the distance as GridSearch's walk finds it
+        const uint4 w = ss.block((uint32_t)k);
+        const float step_size = pp::step_length(pp::u01_open0(w.x), ln2_sca);
+        float d = 0.0f;
+        pp::sphere_hit(p.targets[target], rp, rd, pp::dir_excess(rd), step_size, d);
+        move_by(rp, rdist, rt, rd, d, inv_c);
         r[0] = make_float4(rp.x, rp.y, rp.z, rd.x);
         r[1] = make_float4(rd.y, rd.z, st.dir.x, st.dir.y);
         r[2] = make_float4(st.dir.z, st.wl, rt, rdist);
