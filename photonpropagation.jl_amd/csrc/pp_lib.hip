@@ -344,11 +344,11 @@ __device__ __forceinline__ bool transport_photon(const KParams& p, const pp_sour
 // every lane walks. A job that hits records (step << 16 | target) in its
 // photon's LDS slot with an atomic minimum; the earliest step wins, which is
 // the step at which transport_photon stops (its later steps were never taken
-// by the reference and their jobs only lose the minimum); a photon stops
-// stepping once a hit of its own is known. A hit's record holds the photon id,
-// module, step and the distance within it; replay_hits_kernel then emits the
-// few photons with a hit again and replays their steps up to it (same random
-// stream, same arithmetic) for the rest of the record, so no emission state
+// by the reference and their jobs only lose the minimum), so a photon need
+// not stop at a hit it does not yet know of. A hit's record holds the photon
+// id, module and step; replay_hits_kernel then emits the few photons with a
+// hit again and replays their steps up to it (same random stream, same
+// arithmetic) for the rest of the record, so no emission state
 // stays live across the steps. The result is that of transport_photon bit
 // for bit.
 // ---------------------------------------------------------------------------
@@ -410,10 +410,12 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
     uint32_t ones;
     __asm__ volatile("v_mov_b32 %0, -1" : "=v"(ones));
     *reinterpret_cast<uint2*>(q.key + lane) = make_uint2(ones, ones);
-    // wave masks (scalar): going_m, the lanes still stepping (active, and no
-    // hit found so far; steps after a found hit cannot change the earliest
-    // one); stop_m, the lanes whose hit the last batch made known
-    uint64_t going_m = __ballot(active), stop_m = 0;
+    // the stepping lanes as a wave mask (scalar). A photon whose hit is known
+    // keeps stepping: its later steps cannot change the earliest (step,
+    // target), and with ~1e-3 of the photons hit, stopping them would cost a
+    // read of every lane's key after each batch (an LDS round trip) for
+    // nothing
+    const uint64_t going_m = __ballot(active);
     uint32_t head = 0, count = 0;  // wave-uniform ring of pending jobs
     // every lane walks one pending job (lanes >= n none)
     auto walk_jobs = [&](uint32_t n) {
@@ -436,8 +438,6 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
                 atomicMin(q.key + owner, key);
             }
         }
-        wave_sync();
-        stop_m = __ballot(q.key[lane] != ~0ull);
         head += n;
         if (head >= kQueue) head -= kQueue;
         count -= n;
@@ -449,7 +449,6 @@ __device__ __forceinline__ bool transport_group(const KParams& p, const pp_sourc
     int32_t step = 0;
     bool stepping = true;  // wave-uniform
     for (;;) {
-        going_m &= ~stop_m;
         stepping = stepping && step < p.n_steps && going_m != 0;
         if (!stepping && count == 0) break;
         // r, step_size and next are set and read only while stepping (a
