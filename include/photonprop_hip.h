@@ -146,7 +146,9 @@ enum pp_search_kind { PP_SEARCH_AUTO = 0, PP_SEARCH_BRUTE = 1, PP_SEARCH_GRID = 
  *   PP_BATCH_PER_SOURCE  one launch per source
  *   PP_BATCH_PERSISTENT  one launch for the whole list: waves take chunks of
  *                        cfg.chunk_photons photons of one source from a
- *                        global counter (work stealing)
+ *                        global counter (work stealing); with several
+ *                        sources the last two chunks per resident wave go
+ *                        out in pieces of 256 photons
  *   PP_BATCH_GRAPH       PP_BATCH_PERSISTENT as a hipGraph (counter clear +
  *                        propagation), instantiated once and re-parameterised
  *                        on later calls
