@@ -509,9 +509,11 @@ def _run_cfg(P, wl, seed=None, **cfg_fields):
 
 def test_batch_modes_bit_identical(P, dev):
     """One launch per source, the persistent work-stealing kernel (auto and
-    64-photon chunks; the automatic block size, 768 threads for these 1000
-    modules' tables, and 256) and its hipGraph give the same hits bit for
-    bit, including on a second call that re-parameterises the graph."""
+    64-photon chunks; 960-photon chunks, which a batch this small hands out
+    entirely as pieces of 256, 256, 256 and 192 photons, some past a source's
+    end; the automatic block size, 768 threads for these 1000 modules'
+    tables, and 256) and its hipGraph give the same hits bit for bit,
+    including on a second call that re-parameterises the graph."""
     wl = P.workloads.uneven_batch_1000(n_sources=40, seed=5, min_photons=1e3, max_photons=2e5)
     wl.sources[3].photons = 0
     wl.sources[17].photons = 37  # a partial wave
@@ -523,6 +525,7 @@ def test_batch_modes_bit_identical(P, dev):
         "chunk64": _run_cfg(P, wl, batch_mode=P._lib.BATCH_PERSISTENT, chunk_photons=64,
                             block_size=128),
         "block256": _run_cfg(P, wl, batch_mode=P._lib.BATCH_PERSISTENT, block_size=256),
+        "pieces": _run_cfg(P, wl, batch_mode=P._lib.BATCH_PERSISTENT, chunk_photons=960),
         "graph": _run_cfg(P, wl, batch_mode=P._lib.BATCH_GRAPH, seed=wl.seed + 1),
     }
     runs["graph"], graph2 = runs["graph"]
